@@ -1,0 +1,328 @@
+// kf_api.hip -- C ABI of libkfmi.so (declared in include/kf_abi.h).
+//
+// A kf_ctx owns the device copies of the plane table and the field grid, the
+// per-state store used by the smoother (grown on demand, never shrunk), and
+// two HIP events that time the last fit kernel on its launch stream.
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "../../include/kf_abi.h"
+#include "kf_kernels.hpp"
+
+struct kf_ctx {
+  int device = 0;
+  char err[512] = {0};
+  kfd::DevPlane *planes = nullptr;
+  int32_t n_planes = 0;
+  kfd::DevField field{};
+  bool field_set = false;
+  float *grid = nullptr;
+  size_t grid_elems = 0;
+  double *store = nullptr;  // per-state store
+  size_t store_bytes = 0;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  bool timed = false;
+};
+
+namespace {
+
+char g_create_err[512] = {0};
+
+int fail(kf_ctx *ctx, int code, const char *fmt, ...) {
+  char *buf = ctx ? ctx->err : g_create_err;
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, 512, fmt, ap);
+  va_end(ap);
+  return code;
+}
+
+#define KF_HIP(ctx, call)                                                            \
+  do {                                                                               \
+    hipError_t e_ = (call);                                                          \
+    if (e_ != hipSuccess)                                                            \
+      return fail((ctx), KF_E_HIP, "%s failed: %s", #call, hipGetErrorString(e_));   \
+  } while (0)
+
+// Physics constants evaluated the way the reference's unit literals do
+// (long double literal times a double unit, rounded to double;
+// Interactions.ipp:19-25, Units.hpp:145-153) and its float literals.
+void physicsConstants(kfd::FitArgs &a) {
+  a.kMe = (double)(0.001L * 0.5109989461L);
+  a.kK = ((double)(0.001L * 0.307075L)) * 10.0 * 10.0;
+  a.kPlasma = (double)(1e-9L * 28.816L);
+  a.kHighland = (double)(0.001L * 13.6L);
+  const float c = 0.038f * 2;
+  a.kHighlandLog = (double)c;
+  const float conv = 2 * std::sqrt(2 * std::log(2.0f));
+  a.kLandauConv = (double)conv;
+}
+
+}  // namespace
+
+extern "C" {
+
+int kf_abi_version(void) { return KF_ABI_VERSION; }
+
+void kf_options_default(kf_options *o) {
+  if (!o) return;
+  std::memset(o, 0, sizeof *o);
+  o->smoothing = 1;
+  o->multiple_scattering = 1;
+  o->energy_loss = 1;
+  o->precision = KF_PRECISION_FP64;
+  o->abs_pdg = 13;
+  o->max_steps = 1000;
+  o->max_rk_trials = 10000;
+  o->max_states = 0;
+  o->mass = 105.6583755 * 1e-3;
+  o->tolerance = 1e-4;
+  o->step_size_cutoff = 0.0;
+  o->max_step_size = DBL_MAX;
+  o->target_tolerance = 1e-4;
+  o->overstep_limit = 0.01;
+}
+
+int kf_ctx_create(int device, kf_ctx **out) {
+  if (!out) return fail(nullptr, KF_E_INVALID, "kf_ctx_create: out is NULL");
+  *out = nullptr;
+  int ndev = 0;
+  hipError_t e = hipGetDeviceCount(&ndev);
+  if (e != hipSuccess || ndev <= 0)
+    return fail(nullptr, KF_E_HIP, "kf_ctx_create: no HIP device (%s)", hipGetErrorString(e));
+  if (device < 0 || device >= ndev)
+    return fail(nullptr, KF_E_INVALID, "kf_ctx_create: device %d out of range [0, %d)", device, ndev);
+  kf_ctx *ctx = new (std::nothrow) kf_ctx();
+  if (!ctx) return fail(nullptr, KF_E_NOMEM, "kf_ctx_create: out of host memory");
+  ctx->device = device;
+  e = hipSetDevice(device);
+  if (e == hipSuccess) e = hipEventCreate(&ctx->ev0);
+  if (e == hipSuccess) e = hipEventCreate(&ctx->ev1);
+  if (e != hipSuccess) {
+    fail(nullptr, KF_E_HIP, "kf_ctx_create: %s", hipGetErrorString(e));
+    delete ctx;
+    return KF_E_HIP;
+  }
+  *out = ctx;
+  return KF_OK;
+}
+
+int kf_ctx_destroy(kf_ctx *ctx) {
+  if (!ctx) return KF_OK;
+  (void)hipSetDevice(ctx->device);
+  (void)hipDeviceSynchronize();
+  if (ctx->planes) (void)hipFree(ctx->planes);
+  if (ctx->grid) (void)hipFree(ctx->grid);
+  if (ctx->store) (void)hipFree(ctx->store);
+  if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
+  if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+  delete ctx;
+  return KF_OK;
+}
+
+const char *kf_last_error(const kf_ctx *ctx) { return ctx ? ctx->err : g_create_err; }
+
+int kf_set_geometry(kf_ctx *ctx, const kf_plane *planes, int32_t n_planes) {
+  if (!ctx) return fail(nullptr, KF_E_INVALID, "kf_set_geometry: ctx is NULL");
+  if (!planes || n_planes <= 0)
+    return fail(ctx, KF_E_INVALID, "kf_set_geometry: need n_planes > 0 planes");
+  std::vector<kfd::DevPlane> dp(n_planes);
+  for (int i = 0; i < n_planes; ++i) {
+    const kf_plane &p = planes[i];
+    kfd::DevPlane &d = dp[i];
+    std::memset(&d, 0, sizeof d);
+    for (int k = 0; k < 3; ++k) d.c[k] = p.center[k];
+    for (int k = 0; k < 9; ++k) d.r[k] = p.rot[k];
+    // MaterialSlab (MaterialSlab.ipp:15-20) and Material (Material.ipp:80-85,
+    // Material.hpp:102-104), only when the slab is valid (MaterialSlab.hpp:46-48)
+    const bool valid = p.thickness > 0.0 && 0.0f < p.ar;
+    d.has_material = valid ? 1 : 0;
+    if (valid) {
+      const double eps = 2 * DBL_EPSILON;
+      d.thickness = p.thickness;
+      d.thicknessInX0 = (eps < p.x0) ? (p.thickness / p.x0) : 0.0;
+      d.meanExcitation = (16.0 * 1e-9) * std::pow(p.z, (double)0.9f);
+      d.molarElectronDensity = p.z * p.molar_rho;
+    }
+    d.geo_id = p.geo_id;
+  }
+  KF_HIP(ctx, hipSetDevice(ctx->device));
+  if (ctx->planes && ctx->n_planes != n_planes) {
+    KF_HIP(ctx, hipFree(ctx->planes));
+    ctx->planes = nullptr;
+  }
+  if (!ctx->planes) KF_HIP(ctx, hipMalloc(&ctx->planes, sizeof(kfd::DevPlane) * n_planes));
+  KF_HIP(ctx, hipMemcpy(ctx->planes, dp.data(), sizeof(kfd::DevPlane) * n_planes,
+                        hipMemcpyHostToDevice));
+  ctx->n_planes = n_planes;
+  return KF_OK;
+}
+
+int kf_set_field_constant(kf_ctx *ctx, double bx, double by, double bz) {
+  if (!ctx) return fail(nullptr, KF_E_INVALID, "kf_set_field_constant: ctx is NULL");
+  const double scale = ctx->field_set ? ctx->field.scale : 1.0;
+  ctx->field = kfd::DevField{};
+  ctx->field.grid = 0;
+  ctx->field.b[0] = bx;
+  ctx->field.b[1] = by;
+  ctx->field.b[2] = bz;
+  ctx->field.scale = scale;
+  ctx->field_set = true;
+  return KF_OK;
+}
+
+int kf_set_field_grid(kf_ctx *ctx, const float *b, int32_t nx, int32_t ny, int32_t nz,
+                      const double lo[3], const double hi[3]) {
+  if (!ctx) return fail(nullptr, KF_E_INVALID, "kf_set_field_grid: ctx is NULL");
+  if (!b || !lo || !hi || nx < 2 || ny < 2 || nz < 2)
+    return fail(ctx, KF_E_INVALID, "kf_set_field_grid: need a >= 2x2x2 grid and bounds");
+  for (int d = 0; d < 3; ++d)
+    if (!(hi[d] > lo[d])) return fail(ctx, KF_E_INVALID, "kf_set_field_grid: hi <= lo on axis %d", d);
+  const size_t elems = (size_t)nx * ny * nz * 3;
+  KF_HIP(ctx, hipSetDevice(ctx->device));
+  if (ctx->grid && ctx->grid_elems != elems) {
+    KF_HIP(ctx, hipFree(ctx->grid));
+    ctx->grid = nullptr;
+  }
+  if (!ctx->grid) KF_HIP(ctx, hipMalloc(&ctx->grid, elems * sizeof(float)));
+  ctx->grid_elems = elems;
+  KF_HIP(ctx, hipMemcpy(ctx->grid, b, elems * sizeof(float), hipMemcpyHostToDevice));
+  const double scale = ctx->field_set ? ctx->field.scale : 1.0;
+  ctx->field = kfd::DevField{};
+  ctx->field.grid = 1;
+  ctx->field.values = ctx->grid;
+  ctx->field.n[0] = nx;
+  ctx->field.n[1] = ny;
+  ctx->field.n[2] = nz;
+  const int n[3] = {nx, ny, nz};
+  for (int d = 0; d < 3; ++d) {
+    ctx->field.lo[d] = lo[d];
+    ctx->field.hi[d] = hi[d];
+    ctx->field.w[d] = (hi[d] - lo[d]) / (n[d] - 1);
+  }
+  ctx->field.scale = scale;
+  ctx->field_set = true;
+  return KF_OK;
+}
+
+int kf_set_field_scale(kf_ctx *ctx, double s) {
+  if (!ctx) return fail(nullptr, KF_E_INVALID, "kf_set_field_scale: ctx is NULL");
+  if (!ctx->field_set) return fail(ctx, KF_E_STATE, "kf_set_field_scale: no field set");
+  ctx->field.scale = s;
+  return KF_OK;
+}
+
+int kf_fit(kf_ctx *ctx, const kf_tracks *tr, const kf_options *opt, const kf_results *res,
+           void *stream) {
+  if (!ctx) return fail(nullptr, KF_E_INVALID, "kf_fit: ctx is NULL");
+  if (!tr || !opt || !res) return fail(ctx, KF_E_INVALID, "kf_fit: NULL tracks/options/results");
+  if (!ctx->planes) return fail(ctx, KF_E_STATE, "kf_fit: no geometry (kf_set_geometry)");
+  if (!ctx->field_set) return fail(ctx, KF_E_STATE, "kf_fit: no field (kf_set_field_*)");
+  if (tr->n_tracks < 0 || tr->n_hits < 0) return fail(ctx, KF_E_INVALID, "kf_fit: negative size");
+  if (tr->n_tracks == 0) return KF_OK;
+  if (!tr->start_params || !tr->start_cov || !tr->perigee || !tr->seq_first || !tr->seq_count ||
+      !tr->hit_offset)
+    return fail(ctx, KF_E_INVALID, "kf_fit: NULL per-track input array");
+  if (tr->n_hits > 0 && (!tr->hit_geo_id || !tr->hit_meas || !tr->hit_cov))
+    return fail(ctx, KF_E_INVALID, "kf_fit: NULL hit array");
+  if (!res->params || !res->cov || !res->status)
+    return fail(ctx, KF_E_INVALID, "kf_fit: results need params, cov and status");
+  if (opt->precision != KF_PRECISION_FP64 && opt->precision != KF_PRECISION_MIXED)
+    return fail(ctx, KF_E_INVALID, "kf_fit: unknown precision %d", opt->precision);
+  if (opt->max_steps < 0 || opt->max_rk_trials < 0 || opt->max_states < 0 || !(opt->mass > 0.0) ||
+      !(opt->overstep_limit >= 0.0))
+    return fail(ctx, KF_E_INVALID, "kf_fit: invalid options");
+
+  const int32_t max_states = opt->max_states > 0 ? opt->max_states : ctx->n_planes;
+  kfd::FitArgs a;
+  std::memset(&a, 0, sizeof a);
+  a.n = tr->n_tracks;
+  a.n_hits = tr->n_hits;
+  a.start_params = tr->start_params;
+  a.start_cov = tr->start_cov;
+  a.perigee = tr->perigee;
+  a.seq_first = tr->seq_first;
+  a.seq_count = tr->seq_count;
+  a.hit_offset = tr->hit_offset;
+  a.hit_geo_id = tr->hit_geo_id;
+  a.hit_meas = tr->hit_meas;
+  a.hit_cov = tr->hit_cov;
+  a.planes = ctx->planes;
+  a.n_planes = ctx->n_planes;
+  a.field = ctx->field;
+  a.smoothing = opt->smoothing;
+  a.ms = opt->multiple_scattering;
+  a.eloss = opt->energy_loss;
+  a.max_steps = opt->max_steps;
+  a.max_rk_trials = (uint32_t)opt->max_rk_trials;
+  a.max_states = max_states;
+  a.mass = opt->mass;
+  a.tolerance = opt->tolerance;
+  a.step_size_cutoff = opt->step_size_cutoff;
+  a.max_step_size = opt->max_step_size;
+  a.target_tolerance = opt->target_tolerance;
+  a.overstep_limit = opt->overstep_limit;
+  physicsConstants(a);
+  a.out_params = res->params;
+  a.out_cov = res->cov;
+  a.out_chi2 = res->chi2;
+  a.out_ndf = res->ndf;
+  a.out_nmeas = res->n_measured;
+  a.out_status = res->status;
+
+  KF_HIP(ctx, hipSetDevice(ctx->device));
+  if (res->track_states) {
+    a.ts = res->track_states;
+    a.ts_fields = KF_TS_STRIDE;
+    a.off_pred = KF_TS_PREDICTED;
+    a.off_filt = KF_TS_FILTERED;
+    a.off_jac = KF_TS_JACOBIAN;
+    a.full_record = 1;
+  } else {
+    const int fields = 27 + 27 + 36;
+    const size_t need = (size_t)fields * (size_t)max_states * (size_t)tr->n_tracks * sizeof(double);
+    if (need > ctx->store_bytes) {
+      if (ctx->store) {
+        KF_HIP(ctx, hipStreamSynchronize((hipStream_t)stream));
+        KF_HIP(ctx, hipFree(ctx->store));
+        ctx->store = nullptr;
+        ctx->store_bytes = 0;
+      }
+      hipError_t e = hipMalloc(&ctx->store, need);
+      if (e != hipSuccess)
+        return fail(ctx, KF_E_NOMEM, "kf_fit: cannot allocate %zu bytes of state store: %s", need,
+                    hipGetErrorString(e));
+      ctx->store_bytes = need;
+    }
+    a.ts = ctx->store;
+    a.ts_fields = fields;
+    a.off_pred = 0;
+    a.off_filt = 27;
+    a.off_jac = 54;
+    a.full_record = 0;
+  }
+  hipStream_t s = (hipStream_t)stream;
+  KF_HIP(ctx, hipEventRecord(ctx->ev0, s));
+  hipError_t e = kfd_launch_fit(&a, opt->precision, s);
+  if (e != hipSuccess) return fail(ctx, KF_E_HIP, "kf_fit: launch failed: %s", hipGetErrorString(e));
+  KF_HIP(ctx, hipEventRecord(ctx->ev1, s));
+  ctx->timed = true;
+  return KF_OK;
+}
+
+double kf_last_kernel_ms(const kf_ctx *ctx) {
+  if (!ctx || !ctx->timed) return -1.0;
+  if (hipEventSynchronize(ctx->ev1) != hipSuccess) return -1.0;
+  float ms = 0.0f;
+  if (hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1) != hipSuccess) return -1.0;
+  return (double)ms;
+}
+
+}  // extern "C"

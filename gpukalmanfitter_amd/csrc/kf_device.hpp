@@ -1,0 +1,185 @@
+// kf_device.hpp -- device-side building blocks of the MI355X Kalman fitter.
+//
+// One track per lane.  Every quantity of a track lives in VGPRs; the only
+// memory a lane touches besides its inputs and outputs is its own state store
+// (predicted / filtered / Jacobian per measured surface) that the RTS
+// smoother reads back.
+//
+// Algorithm: the gpuKalmanFitter fit (KalmanFitter.hpp:216-393,716-768 with
+// EigenStepper, DirectNavigator, GainMatrixUpdater, GainMatrixSmoother,
+// PointwiseMaterialInteraction).  The per-lane operation sequence is the
+// reference's; the data layout is ours:
+//  * the free-from-bound Jacobian is accumulated directly, J <- D J, on its
+//    25 non-constant entries (the reference keeps jacTransport = D_n..D_1 and
+//    multiplies once at the surface; same matrix, fewer flops);
+//  * covariances are kept as the 21-entry upper triangle;
+//  * plane and line surfaces are special-cased, no virtual dispatch.
+// The scalar type of the stepping/transport part is a template parameter
+// (double for the plain FP64 fitter, float for the mixed-precision one);
+// positions, directions, momentum, time and the Kalman update are double.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/kf_abi.h"
+
+namespace kfd {
+
+#define KFD_INLINE __device__ __forceinline__
+
+constexpr double kOnSurfaceTolerance = 1e-4;
+constexpr double kOverstepLimit = 0.01;
+constexpr double kPi = 3.14159265358979323846;
+constexpr double kDblMax = 1.7976931348623157e308;
+
+// ---------------------------------------------------------------- plane ----
+// Device plane table entry (built on the host from kf_plane).
+struct DevPlane {
+  double c[3];
+  double r[9];           // row-major rotation, columns U, V, normal
+  double thickness;      // <= 0: no material
+  double thicknessInX0;  // precomputed MaterialSlab ratios
+  double meanExcitation; // 16 eV * Z^0.9f
+  double molarElectronDensity;
+  int32_t geo_id;
+  int32_t has_material;
+};
+
+// ---------------------------------------------------------------- field ----
+struct DevField {
+  int grid;
+  double b[3];
+  double scale;
+  const float *values;
+  int n[3];
+  double lo[3], hi[3], w[3];
+};
+
+// Field lookup in double.  Grid: trilinear, last axis first (the reference's
+// interpolation_impl.hpp:98-131), positions clamped to the box.
+KFD_INLINE void fieldAt(const DevField &f, double x, double y, double z, double B[3]) {
+  if (!f.grid) {
+    if (f.scale == 1.0) {
+      B[0] = f.b[0];
+      B[1] = f.b[1];
+      B[2] = f.b[2];
+    } else {
+      B[0] = f.scale * f.b[0];
+      B[1] = f.scale * f.b[1];
+      B[2] = f.scale * f.b[2];
+    }
+    return;
+  }
+  const double p[3] = {x, y, z};
+  int idx[3];
+  double fr[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    double v = p[d];
+    if (v < f.lo[d]) v = f.lo[d];
+    if (f.hi[d] < v) v = f.hi[d];
+    int i = (int)floor((v - f.lo[d]) / f.w[d]);
+    i = i < 0 ? 0 : i;
+    i = i > f.n[d] - 2 ? f.n[d] - 2 : i;
+    idx[d] = i;
+    const double ll = f.lo[d] + (double)i * f.w[d];
+    const double ur = f.lo[d] + (double)(i + 1) * f.w[d];
+    fr[d] = (v - ll) / (ur - ll);
+  }
+  double cz[4][3];
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    const int ix = idx[0] + (c >> 1), iy = idx[1] + (c & 1);
+    const size_t b0 = (((size_t)ix * f.n[1] + iy) * f.n[2] + idx[2]) * 3;
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+      cz[c][k] = (1.0 - fr[2]) * (double)f.values[b0 + k] + fr[2] * (double)f.values[b0 + 3 + k];
+  }
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const double y0 = (1.0 - fr[1]) * cz[0][k] + fr[1] * cz[1][k];
+    const double y1 = (1.0 - fr[1]) * cz[2][k] + fr[1] * cz[3][k];
+    B[k] = (1.0 - fr[0]) * y0 + fr[0] * y1;
+    if (f.scale != 1.0) B[k] = f.scale * B[k];
+  }
+}
+
+// ------------------------------------------------------- small helpers ----
+template <typename T>
+KFD_INLINE void cross(const T a[3], const T b[3], T r[3]) {
+  r[0] = a[1] * b[2] - a[2] * b[1];
+  r[1] = a[2] * b[0] - a[0] * b[2];
+  r[2] = a[0] * b[1] - a[1] * b[0];
+}
+template <typename T>
+KFD_INLINE T dot3(const T a[3], const T b[3]) {
+  return a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
+}
+
+// packed upper-triangle index of (i, j)
+KFD_INLINE constexpr int sidx(int i, int j) {
+  return (i <= j) ? (i * 6 - (i * (i - 1)) / 2 + (j - i)) : (j * 6 - (j * (j - 1)) / 2 + (i - j));
+}
+
+// phi cyclic in [-pi, pi), theta clamped to [0, pi], each applied twice as
+// ParameterSet::setParameter does (ParameterSet.hpp:276-281).
+KFD_INLINE double cyclicPhi(double x) {
+  const double mn = -kPi, mx = kPi;
+  if (mn <= x && x < mx) return x;
+  return x - (mx - mn) * floor((x - mn) / (mx - mn));
+}
+KFD_INLINE double boundTheta(double x) {
+  return (x > kPi) ? kPi : ((x < 0.0) ? 0.0 : x);
+}
+KFD_INLINE void correctParams(double p[6]) {
+  p[2] = cyclicPhi(cyclicPhi(p[2]));
+  p[3] = boundTheta(boundTheta(p[3]));
+}
+
+// ------------------------------------------------------ ConstrainedStep ----
+// include/Propagator/ConstrainedStep.hpp:22-139
+enum { kAccuracy = 0, kActor = 1, kAborter = 2, kUser = 3 };
+struct CStep {
+  double v[4];
+  int dir;
+  KFD_INLINE void init(double value) {
+    dir = (value > 0.0) ? 1 : -1;
+    v[kAccuracy] = kDblMax * (double)dir;
+    v[kActor] = kDblMax * (double)dir;
+    v[kAborter] = kDblMax * (double)dir;
+    v[kUser] = value;
+  }
+  KFD_INLINE double maxv() const {
+    double m = -kDblMax;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+      if (v[i] > m) m = v[i];
+    return m;
+  }
+  KFD_INLINE double minv() const {
+    double m = kDblMax;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+      if (v[i] < m) m = v[i];
+    return m;
+  }
+  KFD_INLINE void release(int t) {
+    const double r = (dir == 1) ? maxv() : minv();
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+      if (i == t) v[i] = r;
+  }
+  KFD_INLINE void update(double value, int t) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+      if (i == t) v[i] = (fabs(v[i]) < fabs(value)) ? v[i] : value;
+  }
+  KFD_INLINE void assign(double value) {
+    v[kAccuracy] = value;
+    dir = (value > 0.0) ? 1 : -1;
+  }
+  KFD_INLINE double get() const { return (dir == 1) ? minv() : maxv(); }
+};
+
+}  // namespace kfd

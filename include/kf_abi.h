@@ -1,0 +1,197 @@
+/*
+ * kf_abi.h -- C ABI of the MI355X Kalman track fitter (libkfmi.so).
+ *
+ * This is the drop-in boundary for the per-track Kalman fit of
+ * gpuKalmanFitter.  One call of kf_fit() fits a batch of tracks; every entry
+ * point takes plain pointers and sizes, no C++ or torch types.
+ *
+ * Reference interfaces replaced (paths are inside the gpuKalmanFitter tree):
+ *   KalmanFitter::fit(sourcelinks, startParameters, kfOptions, kfResult,
+ *                     surfaceSequence, surfaceSequenceSize) -> bool
+ *       include/Fitter/KalmanFitter.hpp:716-768          -> kf_fit (n_tracks = 1
+ *                                                            is the per-track call)
+ *   fitKernelThreadPerTrack / fitKernelBlockPerTrack
+ *       Run/GPU/Kernels.cu:6-94                           -> kf_fit (batch)
+ *   KalmanFitterOptions{smoothing, multipleScattering, energyLoss, referenceSurface}
+ *       include/Fitter/KalmanFitter.hpp:48-96             -> kf_options
+ *   PropagatorOptions{mass, absPdgCode, maxSteps, maxRungeKuttaStepTrials,
+ *                     maxStepSize, targetTolerance, tolerance, stepSizeCutOff}
+ *       include/Propagator/Propagator.hpp:52-111          -> kf_options
+ *   PlaneSurface(center, normal, HomogeneousSurfaceMaterial) + GeometryID
+ *       include/Surfaces/detail/PlaneSurface.ipp:48-69,
+ *       include/Geometry/GeometryID.hpp:30-123            -> kf_plane, kf_set_geometry
+ *   Test::ConstantBField / InterpolatedBFieldMap::getField
+ *       include/Test/Helper.hpp:17-22,
+ *       include/MagneticField/InterpolatedBFieldMap.hpp:318-320
+ *                                                         -> kf_set_field_constant,
+ *                                                            kf_set_field_grid
+ *   PixelSourceLink{values, cov, geometryId}
+ *       include/EventData/PixelSourceLink.hpp:22-75        -> kf_tracks.hit_*
+ *   BoundState{boundParams, boundCov} + LineSurface(perigee)
+ *       Run/Common/FitData.hpp:45-48, Run/Common/Processor.hpp:121-129
+ *                                                         -> kf_tracks.start_*, perigee
+ *   KalmanFitterResult{fittedStates, fittedParameters, measurementStates,
+ *                      smoothed, finished, result}
+ *       include/Fitter/KalmanFitter.hpp:98-120             -> kf_results
+ *
+ * Units are the reference's native units: mm, GeV, elementary charge, time in
+ * mm/c, magnetic field in GeV/(e*mm) (1 T = 0.000299792458).
+ *
+ * Layout: every per-track array is structure-of-arrays, component k of track i
+ * at [k * n_tracks + i].  Per-hit arrays are CSR over tracks: the hits of
+ * track i are hit_offset[i] .. hit_offset[i+1]-1, component k of hit h at
+ * [k * n_hits + h].  Symmetric 6x6 matrices are packed as the upper triangle,
+ * row-major: (0,0) (0,1) .. (0,5) (1,1) .. (1,5) (2,2) .. (5,5)  -> 21 values.
+ * Bound parameters are (loc0, loc1, phi, theta, q/p, t).
+ *
+ * Errors: every function returns 0 on success and a negative KF_E_* code on
+ * failure; kf_last_error() then holds a message.  Nothing aborts the process.
+ * Per-track outcome is a status bit mask (KF_STATUS_*).
+ */
+#ifndef KF_ABI_H
+#define KF_ABI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define KF_ABI_VERSION 1
+
+/* return codes */
+#define KF_OK 0
+#define KF_E_INVALID (-1)   /* bad argument (null pointer, size, layout) */
+#define KF_E_HIP (-2)       /* a HIP runtime call failed */
+#define KF_E_STATE (-3)     /* context not ready (no geometry / field) */
+#define KF_E_NOMEM (-4)     /* device allocation failed */
+
+/* per-track status bits (kf_results.status) */
+#define KF_STATUS_OK 0x001u          /* fit() returned true: result && all hits used */
+#define KF_STATUS_SMOOTHED 0x002u    /* KalmanFitterResult::smoothed */
+#define KF_STATUS_FINISHED 0x004u    /* KalmanFitterResult::finished (perigee params set) */
+#define KF_STATUS_RESULT_FAIL 0x008u /* KalmanFitterResult::result == false */
+#define KF_STATUS_STEP_FAIL 0x010u   /* an RK step hit the trial limit or step cut-off */
+#define KF_STATUS_MAX_STEPS 0x020u   /* propagation stopped at maxSteps */
+#define KF_STATUS_NAV_BREAK 0x040u   /* surface sequence exhausted before all hits used */
+#define KF_STATUS_SMOOTHER_NAN 0x080u /* smoother gain had a NaN */
+#define KF_STATUS_BAD_INPUT 0x100u   /* empty sequence, too many states, q/p == 0 */
+
+/* kf_options.precision */
+#define KF_PRECISION_FP64 0  /* everything in double */
+#define KF_PRECISION_MIXED 1 /* fp32 stepping / transport, fp64 update + smoother gain */
+
+/* per-state record of kf_results.track_states: component offsets */
+#define KF_TS_PREDICTED 0   /* 6 params + 21 packed cov */
+#define KF_TS_FILTERED 27   /* 6 params + 21 packed cov */
+#define KF_TS_SMOOTHED 54   /* 6 params + 21 packed cov */
+#define KF_TS_JACOBIAN 81   /* 36, row-major bound-to-bound transport from the previous state */
+#define KF_TS_PATH 117      /* path length accumulated since the start surface */
+#define KF_TS_CHI2 118      /* predicted-residual chi2 r^T S^-1 r */
+#define KF_TS_GEO_ID 119    /* geometry id of the surface (as a double) */
+#define KF_TS_STRIDE 120
+
+/* Planar measurement surface with an optional homogeneous material slab.
+ * rot is row-major; its columns are the local U axis, the local V axis and
+ * the plane normal T (PlaneSurface.ipp:48-69 builds U = z x T (or x x T), V = T x U). */
+typedef struct kf_plane {
+  double center[3];
+  double rot[9];
+  double thickness; /* slab thickness [mm]; <= 0: no material */
+  double x0;        /* radiation length [mm] */
+  double l0;        /* nuclear interaction length [mm] */
+  double ar;        /* relative atomic mass */
+  double z;         /* nuclear charge */
+  double molar_rho; /* molar density [mol/mm^3] */
+  double half_u;    /* rectangle half lengths (informational: the direct */
+  double half_v;    /* navigator intersects without a bounds check) */
+  int32_t geo_id;   /* matched against kf_tracks.hit_geo_id */
+  int32_t reserved;
+} kf_plane;
+
+typedef struct kf_options {
+  int32_t smoothing;           /* run the RTS smoother + back-propagation (default 1) */
+  int32_t multiple_scattering; /* default 1 */
+  int32_t energy_loss;         /* default 1 */
+  int32_t precision;           /* KF_PRECISION_* (default FP64) */
+  int32_t abs_pdg;             /* default 13 */
+  int32_t max_steps;           /* default 1000 */
+  int32_t max_rk_trials;       /* default 10000 */
+  int32_t max_states;          /* capacity of the per-track state store; 0 = use the
+                                  longest surface sequence of the batch */
+  double mass;                 /* default 105.6583755 MeV = 0.1056583755 */
+  double tolerance;            /* RK error tolerance, default 1e-4 */
+  double step_size_cutoff;     /* default 0 */
+  double max_step_size;        /* default DBL_MAX */
+  double target_tolerance;     /* default 1e-4 (path-limit aborter) */
+  double overstep_limit;       /* default 0.01 mm: a surface more than this far behind
+                                  is unreachable (EigenStepper.hpp:370, hard-coded
+                                  m_overstepLimit in the reference) */
+} kf_options;
+
+/* One batch of tracks.  Every pointer is a device pointer for kf_fit(). */
+typedef struct kf_tracks {
+  int64_t n_tracks;
+  int64_t n_hits;
+  const double *start_params; /* [6][n_tracks] on the perigee line surface */
+  const double *start_cov;    /* [21][n_tracks] packed */
+  const double *perigee;      /* [3][n_tracks] points of the z-parallel perigee lines */
+  const int32_t *seq_first;   /* [n_tracks] first plane-table index of the sequence */
+  const int32_t *seq_count;   /* [n_tracks] number of planes in the sequence */
+  const int64_t *hit_offset;  /* [n_tracks + 1] CSR offsets into the hit arrays */
+  const int32_t *hit_geo_id;  /* [n_hits] */
+  const double *hit_meas;     /* [2][n_hits] local (u, v) measurement */
+  const double *hit_cov;      /* [3][n_hits] V00, V01, V11 */
+} kf_tracks;
+
+/* Outputs, device pointers.  track_states may be NULL. */
+typedef struct kf_results {
+  double *params;     /* [6][n_tracks] fitted parameters on the perigee line */
+  double *cov;        /* [21][n_tracks] */
+  double *chi2;       /* [n_tracks] sum of predicted-residual chi2 */
+  int32_t *ndf;       /* [n_tracks] 2 * measured (the chi2 includes the start prior) */
+  int32_t *n_measured;/* [n_tracks] KalmanFitterResult::measurementStates */
+  uint32_t *status;   /* [n_tracks] KF_STATUS_* */
+  double *track_states; /* optional [max_states][KF_TS_STRIDE][n_tracks] */
+} kf_results;
+
+typedef struct kf_ctx kf_ctx;
+
+int kf_abi_version(void);
+
+/* Fills *opt with the reference defaults (Propagator.hpp:66-95,
+ * KalmanFitter.hpp:72). */
+void kf_options_default(kf_options *opt);
+
+/* Creates a fitting context on HIP device `device`. */
+int kf_ctx_create(int device, kf_ctx **out);
+int kf_ctx_destroy(kf_ctx *ctx);
+/* Message of the last failing call on ctx (or of kf_ctx_create when ctx is NULL). */
+const char *kf_last_error(const kf_ctx *ctx);
+
+/* Copies the plane table (host memory) to the device. */
+int kf_set_geometry(kf_ctx *ctx, const kf_plane *planes, int32_t n_planes);
+/* Uniform field, GeV/(e*mm). */
+int kf_set_field_constant(kf_ctx *ctx, double bx, double by, double bz);
+/* Trilinear grid field (host memory): b[((ix*ny + iy)*nz + iz)*3 + c] in
+ * GeV/(e*mm) at lo + i*(hi-lo)/(n-1); positions outside are clamped. */
+int kf_set_field_grid(kf_ctx *ctx, const float *b, int32_t nx, int32_t ny,
+                      int32_t nz, const double lo[3], const double hi[3]);
+/* Multiplies the field (constant or grid) by s. */
+int kf_set_field_scale(kf_ctx *ctx, double s);
+
+/* Fits the batch asynchronously on `stream` (a hipStream_t; NULL = default
+ * stream).  Inputs and outputs are device memory owned by the caller. */
+int kf_fit(kf_ctx *ctx, const kf_tracks *tracks, const kf_options *opt,
+           const kf_results *res, void *stream);
+
+/* Duration of the last fit kernel on ctx in milliseconds (HIP events recorded
+ * on the launch stream); < 0 if unknown. */
+double kf_last_kernel_ms(const kf_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* KF_ABI_H */

@@ -1,0 +1,201 @@
+"""CPU oracle of the fitter -- TEST INFRASTRUCTURE ONLY.
+
+ctypes wrappers of liboracle.so (the FP64 C++ restatement of the reference fit
+in kfo_fitter.hpp) and liboracle_count.so (the same source on a flop-counting
+scalar).  Only tests/, __graft_entry__.smoke() and the CPU-baseline leg of
+bench.py use this package, always as the checker or the baseline, never as
+the thing measured or shipped.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+from gpukalmanfitter_amd import abi
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIBS = {}
+
+
+class kfo_field(C.Structure):
+    _fields_ = [("grid", C.c_int32), ("n", C.c_int32 * 3), ("b", C.c_double * 3),
+                ("values", C.c_void_p), ("lo", C.c_double * 3), ("hi", C.c_double * 3),
+                ("scale", C.c_double)]
+
+
+def build():
+    """Compiles liboracle.so / liboracle_count.so (make -C oracle)."""
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def _lib(counting=False):
+    name = "liboracle_count.so" if counting else "liboracle.so"
+    if name in _LIBS:
+        return _LIBS[name]
+    path = os.path.join(_HERE, name)
+    if not os.path.exists(path):
+        build()
+    lib = C.CDLL(path)
+    P = C.c_void_p
+    lib.kfo_fit_batch.argtypes = [P, C.c_int32, C.POINTER(kfo_field), C.POINTER(abi.kf_tracks),
+                                  C.POINTER(abi.kf_options), C.POINTER(abi.kf_results),
+                                  C.c_int32, C.c_int32, P, P, P]
+    lib.kfo_fit_batch.restype = C.c_int
+    lib.kfo_time_fit_batch.argtypes = [P, C.c_int32, C.POINTER(kfo_field), C.POINTER(abi.kf_tracks),
+                                       C.POINTER(abi.kf_options), C.POINTER(abi.kf_results), C.c_int32]
+    lib.kfo_time_fit_batch.restype = C.c_double
+    lib.kfo_propagate.argtypes = [P, C.c_int32, C.POINTER(kfo_field), P, P, P,
+                                  C.POINTER(abi.kf_options), P, P, P, P, P, P]
+    lib.kfo_propagate.restype = C.c_int
+    lib.kfo_gain_update.argtypes = [P, P, P, P, P, P, P]
+    lib.kfo_rts_smooth.argtypes = [C.c_int32, P, P, P, P, P, P, P]
+    lib.kfo_rts_smooth.restype = C.c_int
+    lib.kfo_material.argtypes = [C.c_int32] + [C.c_double] * 9
+    lib.kfo_material.restype = C.c_double
+    lib.kfo_field_get.argtypes = [C.POINTER(kfo_field), P, P]
+    _LIBS[name] = lib
+    return lib
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def make_field(spec):
+    """kfo_field from a generate.FieldSpec (keeps a reference to the grid)."""
+    f = kfo_field()
+    f.scale = float(spec.scale)
+    if spec.kind == "grid":
+        vals = np.ascontiguousarray(spec.values, dtype=np.float32)
+        f.grid = 1
+        for k in range(3):
+            f.n[k] = vals.shape[k]
+            f.lo[k] = spec.lo[k]
+            f.hi[k] = spec.hi[k]
+        f.values = vals.ctypes.data
+        f._keep = vals
+    else:
+        for k in range(3):
+            f.b[k] = spec.b[k]
+    return f
+
+
+class HostResults:
+    """Host output buffers in the kf_results layout."""
+
+    def __init__(self, n, max_states=0):
+        self.params = np.zeros((6, n))
+        self.cov = np.zeros((21, n))
+        self.chi2 = np.zeros(n)
+        self.ndf = np.zeros(n, dtype=np.int32)
+        self.n_measured = np.zeros(n, dtype=np.int32)
+        self.status = np.zeros(n, dtype=np.uint32)
+        self.track_states = (np.zeros((max_states, abi.KF_TS_STRIDE, n))
+                             if max_states > 0 else None)
+
+    def c(self):
+        r = abi.kf_results()
+        for name in ("params", "cov", "chi2", "ndf", "n_measured", "status"):
+            setattr(r, name, getattr(self, name).ctypes.data)
+        r.track_states = self.track_states.ctypes.data if self.track_states is not None else None
+        return r
+
+
+def host_tracks(batch):
+    t = abi.kf_tracks()
+    t.n_tracks = batch.n_tracks
+    t.n_hits = batch.n_hits
+    keep = []
+    for name in ("start_params", "start_cov", "perigee", "seq_first", "seq_count", "hit_offset",
+                 "hit_geo_id", "hit_meas", "hit_cov"):
+        arr = np.ascontiguousarray(getattr(batch, name))
+        keep.append(arr)
+        setattr(t, name, arr.ctypes.data)
+    t._keep = keep
+    return t
+
+
+def fit(batch, options=None, track_states=False, n_threads=0, counting=False, planes=None,
+        field=None):
+    """Fits ``batch`` with the oracle.  Returns (HostResults, steps, flops, trans)."""
+    lib = _lib(counting)
+    opts = options if options is not None else abi.default_options()
+    n = batch.n_tracks
+    max_states = int(opts.max_states) if opts.max_states > 0 else int(batch.max_states)
+    res = HostResults(n, max_states if track_states else 0)
+    steps = np.zeros(n, dtype=np.int32)
+    flops = np.zeros(n, dtype=np.uint64)
+    trans = np.zeros(n, dtype=np.uint64)
+    pl = planes if planes is not None else batch.planes
+    fl = make_field(field if field is not None else batch.field)
+    tr = host_tracks(batch)
+    rc = lib.kfo_fit_batch(pl, len(pl), C.byref(fl), C.byref(tr), C.byref(opts), C.byref(res.c()),
+                           max_states, n_threads, _p(steps), _p(flops), _p(trans))
+    if rc != 0:
+        raise RuntimeError(f"kfo_fit_batch failed ({rc})")
+    return res, steps, flops, trans
+
+
+def time_fit(batch, options=None, n_threads=0):
+    """Seconds spent in the oracle's fit loop (OpenMP over tracks)."""
+    lib = _lib(False)
+    opts = options if options is not None else abi.default_options()
+    res = HostResults(batch.n_tracks)
+    fl = make_field(batch.field)
+    tr = host_tracks(batch)
+    return lib.kfo_time_fit_batch(batch.planes, len(batch.planes), C.byref(fl), C.byref(tr),
+                                  C.byref(opts), C.byref(res.c()), n_threads)
+
+
+def propagate(planes, field, start_par, start_cov, perigee, options=None):
+    """Propagation without fitting; bound state at every plane reached."""
+    lib = _lib(False)
+    opts = options if options is not None else abi.default_options()
+    n = len(planes)
+    out = dict(params=np.zeros((n, 6)), cov=np.zeros((n, 6, 6)), jac=np.zeros((n, 6, 6)),
+               pos=np.zeros((n, 3)), dir=np.zeros((n, 3)), path=np.zeros(n))
+    sp = np.ascontiguousarray(start_par, dtype=np.float64)
+    sc = np.ascontiguousarray(start_cov, dtype=np.float64).reshape(36)
+    pc = np.ascontiguousarray(perigee, dtype=np.float64)
+    fl = make_field(field)
+    k = lib.kfo_propagate(planes, n, C.byref(fl), _p(sp), _p(sc), _p(pc), C.byref(opts),
+                          _p(out["params"]), _p(out["cov"]), _p(out["jac"]), _p(out["pos"]),
+                          _p(out["dir"]), _p(out["path"]))
+    return {key: v[:k] for key, v in out.items()}
+
+
+def gain_update(pred, pcov, meas, V):
+    lib = _lib(False)
+    f = np.zeros(6)
+    fc = np.zeros((6, 6))
+    chi2 = C.c_double()
+    lib.kfo_gain_update(_p(np.ascontiguousarray(pred, dtype=np.float64)),
+                        _p(np.ascontiguousarray(pcov, dtype=np.float64)),
+                        _p(np.ascontiguousarray(meas, dtype=np.float64)),
+                        _p(np.ascontiguousarray(V, dtype=np.float64)), _p(f), _p(fc),
+                        C.byref(chi2))
+    return f, fc, chi2.value
+
+
+def rts_smooth(fpar, fcov, ppar, pcov, jac):
+    lib = _lib(False)
+    n = len(fpar)
+    sp = np.zeros((n, 6))
+    sc = np.zeros((n, 6, 6))
+    args = [np.ascontiguousarray(x, dtype=np.float64) for x in (fpar, fcov, ppar, pcov, jac)]
+    rc = lib.kfo_rts_smooth(n, *[_p(a) for a in args], _p(sp), _p(sc))
+    return rc, sp, sc
+
+
+def material(which, slab, mass, qop, q):
+    """which: 'bethe' | 'highland' | 'landau_qop'; slab = (thickness, x0, l0, ar, z, molar_rho)."""
+    w = {"bethe": 0, "highland": 1, "landau_qop": 2}[which]
+    return _lib(False).kfo_material(w, *map(float, slab), float(mass), float(qop), float(q))
+
+
+def field_get(field, pos):
+    fl = make_field(field)
+    b = np.zeros(3)
+    _lib(False).kfo_field_get(C.byref(fl), _p(np.ascontiguousarray(pos, dtype=np.float64)), _p(b))
+    return b
