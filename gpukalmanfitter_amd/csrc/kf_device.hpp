@@ -26,7 +26,7 @@
 
 namespace kfd {
 
-#define KFD_INLINE __device__ __forceinline__
+#define KFD_INLINE __host__ __device__ __forceinline__
 
 constexpr double kOnSurfaceTolerance = 1e-4;
 constexpr double kOverstepLimit = 0.01;

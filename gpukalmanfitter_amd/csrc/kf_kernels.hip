@@ -226,8 +226,10 @@ struct TrackFit {
       err = h2 * e1;
       err = (err < 1e-20) ? 1e-20 : err;
       if (err <= a.tolerance) break;
+      // std::min(std::max(0.25, x), 4.) with its NaN behaviour: a NaN error
+      // estimate (h = DBL_MAX after an actor release) scales by 0.25
       double sc = pow(a.tolerance / fabs(2.0 * err), 0.25);
-      sc = (sc < 0.25) ? 0.25 : sc;
+      sc = (0.25 < sc) ? sc : 0.25;
       sc = (4.0 < sc) ? 4.0 : sc;
       ss.assign(ss.get() * sc);
       const double hn = ss.get();
@@ -754,116 +756,154 @@ struct TrackFit {
   }
 
   // -------------------------------------------------------------- smoother ----
-  // GainMatrixSmoother (GainMatrixSmoother.hpp:21-94): G = C_f J^T C_p^-1,
-  // with C_p^-1 applied through a Cholesky solve.
+  // 6x6 inverse by LU decomposition with partial pivoting, the method of the
+  // reference's host path (Eigen inverse() in calculateInverse,
+  // include/Utilities/Math.hpp:103-112).  Row swaps are predicated selects so
+  // the matrix stays in registers.
+  KFD_INLINE static void inverse6(double lu[6][6], double inv[6][6]) {
+    int perm[6] = {0, 1, 2, 3, 4, 5};
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+      int pv = k;
+      double best = fabs(lu[k][k]);
+#pragma unroll
+      for (int r = k + 1; r < 6; ++r) {
+        const double v = fabs(lu[r][k]);
+        if (best < v) {
+          best = v;
+          pv = r;
+        }
+      }
+#pragma unroll
+      for (int r = k + 1; r < 6; ++r) {
+        const bool sw = (r == pv);
+#pragma unroll
+        for (int c = 0; c < 6; ++c) {
+          const double t = lu[k][c];
+          lu[k][c] = sw ? lu[r][c] : lu[k][c];
+          lu[r][c] = sw ? t : lu[r][c];
+        }
+        const int tp = perm[k];
+        perm[k] = sw ? perm[r] : perm[k];
+        perm[r] = sw ? tp : perm[r];
+      }
+#pragma unroll
+      for (int r = k + 1; r < 6; ++r) {
+        lu[r][k] = lu[r][k] / lu[k][k];
+#pragma unroll
+        for (int c = k + 1; c < 6; ++c) lu[r][c] = lu[r][c] - lu[r][k] * lu[k][c];
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < 6; ++c) {
+      double y[6];
+#pragma unroll
+      for (int r = 0; r < 6; ++r) {
+        double s = (perm[r] == c) ? 1.0 : 0.0;
+#pragma unroll
+        for (int k = 0; k < r; ++k) s = s - lu[r][k] * y[k];
+        y[r] = s;
+      }
+#pragma unroll
+      for (int r = 5; r >= 0; --r) {
+        double s = y[r];
+#pragma unroll
+        for (int k = r + 1; k < 6; ++k) s = s - lu[r][k] * inv[k][c];
+        inv[r][c] = s / lu[r][r];
+      }
+    }
+  }
+
+  // GainMatrixSmoother (GainMatrixSmoother.hpp:21-94):
+  // G = (C_f J^T) C_p^-1, x_s = x_f + G (x_s' - x_p'), C_s = C_f - G (C_p - C_s') G^T
   KFD_INLINE bool smooth(double xs[6], double cs[21]) {
     const int n = measured;
     double xsn[6], csn[21];
     loadBound(n - 1, a.off_filt, xsn, csn);  // last state: smoothed = filtered
     if (a.full_record) storeBound(n - 1, KF_TS_SMOOTHED, xsn, csn);
     for (int k = n - 2; k >= 0; --k) {
-      double xpn[6], cpn[21], xf[6], cfk[21], J[6][6];
-      loadBound(k + 1, a.off_pred, xpn, cpn);
-      loadBound(k, a.off_filt, xf, cfk);
+      double G[6][6];
+      {
+        double inv[6][6];
+        {
+          double lu[6][6];
 #pragma unroll
-      for (int r = 0; r < 6; ++r)
+          for (int r = 0; r < 6; ++r)
 #pragma unroll
-        for (int c = 0; c < 6; ++c) J[r][c] = *rec(k + 1, a.off_jac + 6 * r + c);
-      // M = J C_f   (6x6);  G^T = C_p^-1 M
-      double M[6][6];
-#pragma unroll
-      for (int r = 0; r < 6; ++r)
-#pragma unroll
-        for (int c = 0; c < 6; ++c) {
-          double s = J[r][0] * cfk[sidx(0, c)];
-#pragma unroll
-          for (int m = 1; m < 6; ++m) s = s + J[r][m] * cfk[sidx(m, c)];
-          M[r][c] = s;
+            for (int c = 0; c < 6; ++c) lu[r][c] = *rec(k + 1, a.off_pred + 6 + sidx(r, c));
+          inverse6(lu, inv);
         }
-      // Cholesky of C_p (lower L, row-major), then solve L L^T X = M
-      double L[6][6];
-      bool bad = false;
+        double A1[6][6];  // C_f J^T
+        {
+          double J[6][6];
 #pragma unroll
-      for (int r = 0; r < 6; ++r) {
+          for (int r = 0; r < 6; ++r)
 #pragma unroll
-        for (int c = 0; c <= r; ++c) {
-          double s = cpn[sidx(r, c)];
+            for (int c = 0; c < 6; ++c) J[r][c] = *rec(k + 1, a.off_jac + 6 * r + c);
 #pragma unroll
-          for (int m = 0; m < c; ++m) s = s - L[r][m] * L[c][m];
-          if (r == c) {
-            bad = bad || !(s > 0.0);
-            L[r][r] = sqrt(s);
-          } else {
-            L[r][c] = s / L[c][c];
+          for (int r = 0; r < 6; ++r)
+#pragma unroll
+            for (int c = 0; c < 6; ++c) {
+              double s = *rec(k, a.off_filt + 6 + sidx(r, 0)) * J[c][0];
+#pragma unroll
+              for (int m = 1; m < 6; ++m) s = s + *rec(k, a.off_filt + 6 + sidx(r, m)) * J[c][m];
+              A1[r][c] = s;
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < 6; ++r)
+#pragma unroll
+          for (int c = 0; c < 6; ++c) {
+            double s = A1[r][0] * inv[0][c];
+#pragma unroll
+            for (int m = 1; m < 6; ++m) s = s + A1[r][m] * inv[m][c];
+            G[r][c] = s;
           }
-        }
       }
-      double X[6][6];
-#pragma unroll
-      for (int c = 0; c < 6; ++c) {
-        double y[6];
-#pragma unroll
-        for (int r = 0; r < 6; ++r) {
-          double s = M[r][c];
-#pragma unroll
-          for (int m = 0; m < r; ++m) s = s - L[r][m] * y[m];
-          y[r] = s / L[r][r];
-        }
-#pragma unroll
-        for (int r = 5; r >= 0; --r) {
-          double s = y[r];
-#pragma unroll
-          for (int m = r + 1; m < 6; ++m) s = s - L[m][r] * X[m][c];
-          X[r][c] = s / L[r][r];
-        }
-      }
-      // G = X^T;  NaN check as GainMatrixSmoother.hpp:63-66
-      bool nan = bad;
+      bool nan = false;  // GainMatrixSmoother.hpp:63-66
 #pragma unroll
       for (int r = 0; r < 6; ++r)
 #pragma unroll
-        for (int c = 0; c < 6; ++c) nan = nan || isnan(X[r][c]);
+        for (int c = 0; c < 6; ++c) nan = nan || (G[r][c] != G[r][c]);
       if (nan) {
         smootherNaN = true;
         return false;
       }
+      double xpn[6], cpn[21];
+      loadBound(k + 1, a.off_pred, xpn, cpn);
       double dx[6];
 #pragma unroll
       for (int r = 0; r < 6; ++r) dx[r] = xsn[r] - xpn[r];
-      double xsk[6];
 #pragma unroll
       for (int r = 0; r < 6; ++r) {
-        double s = X[0][r] * dx[0];
+        double s = G[r][0] * dx[0];
 #pragma unroll
-        for (int m = 1; m < 6; ++m) s = s + X[m][r] * dx[m];
-        xsk[r] = xf[r] + s;
+        for (int m = 1; m < 6; ++m) s = s + G[r][m] * dx[m];
+        xsn[r] = *rec(k, a.off_filt + r) + s;
       }
-      // C_s = C_f - G (C_p - C_s,next) G^T
       double Dm[21];
 #pragma unroll
       for (int m = 0; m < 21; ++m) Dm[m] = cpn[m] - csn[m];
-      double GD[6][6];
+      double B1[6][6];  // G (C_p - C_s')
 #pragma unroll
       for (int r = 0; r < 6; ++r)
 #pragma unroll
         for (int c = 0; c < 6; ++c) {
-          double s = X[0][r] * Dm[sidx(0, c)];
+          double s = G[r][0] * Dm[sidx(0, c)];
 #pragma unroll
-          for (int m = 1; m < 6; ++m) s = s + X[m][r] * Dm[sidx(m, c)];
-          GD[r][c] = s;
+          for (int m = 1; m < 6; ++m) s = s + G[r][m] * Dm[sidx(m, c)];
+          B1[r][c] = s;
         }
 #pragma unroll
       for (int r = 0; r < 6; ++r)
 #pragma unroll
         for (int c = r; c < 6; ++c) {
-          double s = GD[r][0] * X[0][c];
+          double s = B1[r][0] * G[c][0];
 #pragma unroll
-          for (int m = 1; m < 6; ++m) s = s + GD[r][m] * X[m][c];
-          csn[sidx(r, c)] = cfk[sidx(r, c)] - s;
+          for (int m = 1; m < 6; ++m) s = s + B1[r][m] * G[c][m];
+          csn[sidx(r, c)] = *rec(k, a.off_filt + 6 + sidx(r, c)) - s;
         }
-      correctParams(xsk);
-#pragma unroll
-      for (int m = 0; m < 6; ++m) xsn[m] = xsk[m];
+      correctParams(xsn);
       if (a.full_record) storeBound(k, KF_TS_SMOOTHED, xsn, csn);
     }
 #pragma unroll

