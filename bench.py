@@ -48,7 +48,22 @@ def parse():
     ap.add_argument("--cpu-sample", type=int, default=1048576,
                     help="tracks of the same workload timed on the CPU oracle")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only copy a MIB-sized device buffer (calibrates rocprofv3 "
+                         "FETCH_SIZE / WRITE_SIZE against known bytes) and exit")
     return ap.parse_args()
+
+
+def copy_calibration(mib):
+    """One device-to-device copy of `mib` MiB: reads and writes mib MiB of HBM."""
+    import torch
+    n = mib * (1 << 20) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda:0")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes": n * 4}), flush=True)
 
 
 def cpu_baseline(batch, opts_c, threads, sample):
@@ -70,6 +85,9 @@ def cpu_baseline(batch, opts_c, threads, sample):
 
 def main():
     args = parse()
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (got --gpus {args.gpus}); "
               "tracks are independent, see DESIGN.md for how the batch would shard", file=sys.stderr)

@@ -27,6 +27,8 @@ struct kf_ctx {
   size_t grid_elems = 0;
   double *store = nullptr;  // per-state store
   size_t store_bytes = 0;
+  void *carry = nullptr;    // forward -> backward kernel hand-over
+  size_t carry_bytes = 0;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   bool timed = false;
 };
@@ -121,6 +123,7 @@ int kf_ctx_destroy(kf_ctx *ctx) {
   if (ctx->planes) (void)hipFree(ctx->planes);
   if (ctx->grid) (void)hipFree(ctx->grid);
   if (ctx->store) (void)hipFree(ctx->store);
+  if (ctx->carry) (void)hipFree(ctx->carry);
   if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
   if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
   delete ctx;
@@ -278,16 +281,11 @@ int kf_fit(kf_ctx *ctx, const kf_tracks *tr, const kf_options *opt, const kf_res
   a.out_status = res->status;
 
   KF_HIP(ctx, hipSetDevice(ctx->device));
-  if (res->track_states) {
-    a.ts = res->track_states;
-    a.ts_fields = KF_TS_STRIDE;
-    a.off_pred = KF_TS_PREDICTED;
-    a.off_filt = KF_TS_FILTERED;
-    a.off_jac = KF_TS_JACOBIAN;
-    a.full_record = 1;
-  } else {
-    const int fields = 27 + 27 + 36;
-    const size_t need = (size_t)fields * (size_t)max_states * (size_t)tr->n_tracks * sizeof(double);
+  {
+    // state store, chunked by 64 tracks (see kf_kernels.hpp)
+    const int fields = res->track_states ? KF_TS_STRIDE : (27 + 27 + 36);
+    const size_t chunks = ((size_t)tr->n_tracks + 63) / 64;
+    const size_t need = chunks * 64 * (size_t)fields * (size_t)max_states * sizeof(double);
     if (need > ctx->store_bytes) {
       if (ctx->store) {
         KF_HIP(ctx, hipStreamSynchronize((hipStream_t)stream));
@@ -303,14 +301,41 @@ int kf_fit(kf_ctx *ctx, const kf_tracks *tr, const kf_options *opt, const kf_res
     }
     a.ts = ctx->store;
     a.ts_fields = fields;
-    a.off_pred = 0;
-    a.off_filt = 27;
-    a.off_jac = 54;
-    a.full_record = 0;
+    if (res->track_states) {
+      a.off_pred = KF_TS_PREDICTED;
+      a.off_filt = KF_TS_FILTERED;
+      a.off_jac = KF_TS_JACOBIAN;
+      a.full_record = 1;
+    } else {
+      a.off_pred = 0;
+      a.off_filt = 27;
+      a.off_jac = 54;
+      a.full_record = 0;
+    }
+  }
+  if (res->track_states && !res->n_measured)
+    return fail(ctx, KF_E_INVALID, "kf_fit: track_states needs results.n_measured");
+  {
+    const size_t need = (size_t)tr->n_tracks * (6 * sizeof(int32_t) + 30 * sizeof(double));
+    if (need > ctx->carry_bytes) {
+      if (ctx->carry) {
+        KF_HIP(ctx, hipStreamSynchronize((hipStream_t)stream));
+        KF_HIP(ctx, hipFree(ctx->carry));
+        ctx->carry = nullptr;
+        ctx->carry_bytes = 0;
+      }
+      hipError_t e = hipMalloc(&ctx->carry, need);
+      if (e != hipSuccess)
+        return fail(ctx, KF_E_NOMEM, "kf_fit: cannot allocate %zu bytes of hand-over buffer: %s",
+                    need, hipGetErrorString(e));
+      ctx->carry_bytes = need;
+    }
+    a.carry_d = (double *)ctx->carry;
+    a.carry_i = (int32_t *)((char *)ctx->carry + (size_t)tr->n_tracks * 30 * sizeof(double));
   }
   hipStream_t s = (hipStream_t)stream;
   KF_HIP(ctx, hipEventRecord(ctx->ev0, s));
-  hipError_t e = kfd_launch_fit(&a, opt->precision, s);
+  hipError_t e = kfd_launch_fit(&a, opt->precision, s, res->track_states);
   if (e != hipSuccess) return fail(ctx, KF_E_HIP, "kf_fit: launch failed: %s", hipGetErrorString(e));
   KF_HIP(ctx, hipEventRecord(ctx->ev1, s));
   ctx->timed = true;

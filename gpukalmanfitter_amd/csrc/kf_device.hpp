@@ -28,6 +28,10 @@ namespace kfd {
 
 #define KFD_INLINE __host__ __device__ __forceinline__
 
+// Keeps the scheduler from hoisting the next phase's memory loads above this
+// point (bounds the number of live registers in load-heavy phases).
+#define KFD_PHASE_FENCE() __asm__ volatile("" ::: "memory")
+
 constexpr double kOnSurfaceTolerance = 1e-4;
 constexpr double kOverstepLimit = 0.01;
 constexpr double kPi = 3.14159265358979323846;

@@ -15,7 +15,13 @@
 
 namespace kfd {
 
-template <typename S>
+// FWD = true: the forward kernel (start, steps, filter at every surface) that
+// stops where the reference's actor would call finalize() and saves a carry
+// record; FWD = false: the backward kernel (smoother, then propagation back to
+// the perigee) that resumes from that record.
+enum { kResumeNone = 0, kResumePreLoop = 1, kResumeInLoop = 2, kResumePostLoop = 3 };
+
+template <typename S, bool FWD>
 struct TrackFit {
   const FitArgs &a;
   const int64_t i;
@@ -54,11 +60,19 @@ struct TrackFit {
   double fpar[6];
   double fcov[21];
 
-  KFD_INLINE TrackFit(const FitArgs &args, int64_t idx) : a(args), i(idx) {}
+  // per-state store of this wave's 64-track chunk (wave-uniform base) and
+  // this lane's slot in it
+  double *tsb;
+  uint32_t lane;
+
+  KFD_INLINE TrackFit(const FitArgs &args, int64_t idx, double *chunk_base, uint32_t ln)
+      : a(args), i(idx), tsb(chunk_base), lane(ln) {}
 
   // ------------------------------------------------------------ storage ----
+  // Element f of state k: tsb[(k * F + f) * 64 + lane] -- a 32-bit lane
+  // offset from a wave-uniform base, so each access is one saddr load/store.
   KFD_INLINE double *rec(int k, int f) const {
-    return a.ts + ((size_t)k * (size_t)a.ts_fields + (size_t)f) * (size_t)a.n + (size_t)i;
+    return tsb + (uint32_t)(((uint32_t)k * (uint32_t)a.ts_fields + (uint32_t)f) * 64u + lane);
   }
   KFD_INLINE void storeBound(int k, int off, const double par[6], const double c[21]) const {
 #pragma unroll
@@ -391,26 +405,41 @@ struct TrackFit {
       jb[4][c] = jp[7];
       jb[5][c] = jp[3];
     }
-    // cov <- jb cov jb^T
-    S T[6][6];
+    // cov <- jb cov jb^T.  Row 4 of jb is e4 and column 5 is e5 exactly (the
+    // q/p row and the time column of the transport), so those terms are
+    // skipped; row r of jb cov is formed and consumed at once.
+    double cn[21];
 #pragma unroll
-    for (int r = 0; r < 6; ++r)
+    for (int r = 0; r < 6; ++r) {
+      S T[6];
 #pragma unroll
       for (int k = 0; k < 6; ++k) {
-        S s = jb[r][0] * (S)cov[sidx(0, k)];
+        if (r == 4) {
+          T[k] = (S)cov[sidx(4, k)];
+        } else {
+          S s = jb[r][0] * (S)cov[sidx(0, k)];
 #pragma unroll
-        for (int c = 1; c < 6; ++c) s = s + jb[r][c] * (S)cov[sidx(c, k)];
-        T[r][k] = s;
+          for (int c = 1; c < 5; ++c) s = s + jb[r][c] * (S)cov[sidx(c, k)];
+          if (r == 5) s = s + (S)cov[sidx(5, k)];
+          T[k] = s;
+        }
       }
-#pragma unroll
-    for (int r = 0; r < 6; ++r)
 #pragma unroll
       for (int c = r; c < 6; ++c) {
-        S s = T[r][0] * jb[c][0];
+        S s;
+        if (c == 4) {
+          s = T[4];
+        } else {
+          s = T[0] * jb[c][0];
 #pragma unroll
-        for (int k = 1; k < 6; ++k) s = s + T[r][k] * jb[c][k];
-        cov[sidx(r, c)] = (double)s;
+          for (int k = 1; k < 5; ++k) s = s + T[k] * jb[c][k];
+          if (c == 5) s = s + T[5];
+        }
+        cn[sidx(r, c)] = (double)s;
       }
+    }
+#pragma unroll
+    for (int m = 0; m < 21; ++m) cov[m] = cn[m];
   }
 
   // boundState on a plane (EigenStepper.ipp:516-540, CovarianceEngine.hpp:
@@ -756,12 +785,13 @@ struct TrackFit {
   }
 
   // -------------------------------------------------------------- smoother ----
-  // 6x6 inverse by LU decomposition with partial pivoting, the method of the
-  // reference's host path (Eigen inverse() in calculateInverse,
+  // LU decomposition with partial pivoting of a 6x6 matrix in place, the
+  // method of the reference's host path (Eigen inverse() in calculateInverse,
   // include/Utilities/Math.hpp:103-112).  Row swaps are predicated selects so
   // the matrix stays in registers.
-  KFD_INLINE static void inverse6(double lu[6][6], double inv[6][6]) {
-    int perm[6] = {0, 1, 2, 3, 4, 5};
+  KFD_INLINE static void lu6(double lu[6][6], int perm[6]) {
+#pragma unroll
+    for (int k = 0; k < 6; ++k) perm[k] = k;
 #pragma unroll
     for (int k = 0; k < 6; ++k) {
       int pv = k;
@@ -794,23 +824,25 @@ struct TrackFit {
         for (int c = k + 1; c < 6; ++c) lu[r][c] = lu[r][c] - lu[r][k] * lu[k][c];
       }
     }
+  }
+
+  // Column c of the inverse from the LU factors (forward then back
+  // substitution on the permuted unit vector).
+  KFD_INLINE static void invColumn(const double lu[6][6], const int perm[6], int c, double x[6]) {
+    double y[6];
 #pragma unroll
-    for (int c = 0; c < 6; ++c) {
-      double y[6];
+    for (int r = 0; r < 6; ++r) {
+      double s = (perm[r] == c) ? 1.0 : 0.0;
 #pragma unroll
-      for (int r = 0; r < 6; ++r) {
-        double s = (perm[r] == c) ? 1.0 : 0.0;
+      for (int k = 0; k < r; ++k) s = s - lu[r][k] * y[k];
+      y[r] = s;
+    }
 #pragma unroll
-        for (int k = 0; k < r; ++k) s = s - lu[r][k] * y[k];
-        y[r] = s;
-      }
+    for (int r = 5; r >= 0; --r) {
+      double s = y[r];
 #pragma unroll
-      for (int r = 5; r >= 0; --r) {
-        double s = y[r];
-#pragma unroll
-        for (int k = r + 1; k < 6; ++k) s = s - lu[r][k] * inv[k][c];
-        inv[r][c] = s / lu[r][r];
-      }
+      for (int k = r + 1; k < 6; ++k) s = s - lu[r][k] * x[k];
+      x[r] = s / lu[r][r];
     }
   }
 
@@ -824,41 +856,41 @@ struct TrackFit {
     for (int k = n - 2; k >= 0; --k) {
       double G[6][6];
       {
-        double inv[6][6];
-        {
-          double lu[6][6];
-#pragma unroll
-          for (int r = 0; r < 6; ++r)
-#pragma unroll
-            for (int c = 0; c < 6; ++c) lu[r][c] = *rec(k + 1, a.off_pred + 6 + sidx(r, c));
-          inverse6(lu, inv);
-        }
-        double A1[6][6];  // C_f J^T
-        {
-          double J[6][6];
-#pragma unroll
-          for (int r = 0; r < 6; ++r)
-#pragma unroll
-            for (int c = 0; c < 6; ++c) J[r][c] = *rec(k + 1, a.off_jac + 6 * r + c);
-#pragma unroll
-          for (int r = 0; r < 6; ++r)
-#pragma unroll
-            for (int c = 0; c < 6; ++c) {
-              double s = *rec(k, a.off_filt + 6 + sidx(r, 0)) * J[c][0];
-#pragma unroll
-              for (int m = 1; m < 6; ++m) s = s + *rec(k, a.off_filt + 6 + sidx(r, m)) * J[c][m];
-              A1[r][c] = s;
-            }
-        }
+        double lu[6][6];
+        int perm[6];
 #pragma unroll
         for (int r = 0; r < 6; ++r)
 #pragma unroll
-          for (int c = 0; c < 6; ++c) {
-            double s = A1[r][0] * inv[0][c];
+          for (int c = 0; c < 6; ++c) lu[r][c] = *rec(k + 1, a.off_pred + 6 + sidx(r, c));
+        lu6(lu, perm);
+        KFD_PHASE_FENCE();
+        double A1[6][6];  // C_f J^T
 #pragma unroll
-            for (int m = 1; m < 6; ++m) s = s + A1[r][m] * inv[m][c];
+        for (int c = 0; c < 6; ++c) {
+          double jr[6];
+#pragma unroll
+          for (int m = 0; m < 6; ++m) jr[m] = *rec(k + 1, a.off_jac + 6 * c + m);
+#pragma unroll
+          for (int r = 0; r < 6; ++r) {
+            double s = *rec(k, a.off_filt + 6 + sidx(r, 0)) * jr[0];
+#pragma unroll
+            for (int m = 1; m < 6; ++m) s = s + *rec(k, a.off_filt + 6 + sidx(r, m)) * jr[m];
+            A1[r][c] = s;
+          }
+        }
+        KFD_PHASE_FENCE();
+#pragma unroll
+        for (int c = 0; c < 6; ++c) {
+          double ic[6];
+          invColumn(lu, perm, c, ic);
+#pragma unroll
+          for (int r = 0; r < 6; ++r) {
+            double s = A1[r][0] * ic[0];
+#pragma unroll
+            for (int m = 1; m < 6; ++m) s = s + A1[r][m] * ic[m];
             G[r][c] = s;
           }
+        }
       }
       bool nan = false;  // GainMatrixSmoother.hpp:63-66
 #pragma unroll
@@ -869,11 +901,10 @@ struct TrackFit {
         smootherNaN = true;
         return false;
       }
-      double xpn[6], cpn[21];
-      loadBound(k + 1, a.off_pred, xpn, cpn);
+      KFD_PHASE_FENCE();
       double dx[6];
 #pragma unroll
-      for (int r = 0; r < 6; ++r) dx[r] = xsn[r] - xpn[r];
+      for (int r = 0; r < 6; ++r) dx[r] = xsn[r] - *rec(k + 1, a.off_pred + r);
 #pragma unroll
       for (int r = 0; r < 6; ++r) {
         double s = G[r][0] * dx[0];
@@ -881,28 +912,28 @@ struct TrackFit {
         for (int m = 1; m < 6; ++m) s = s + G[r][m] * dx[m];
         xsn[r] = *rec(k, a.off_filt + r) + s;
       }
+      KFD_PHASE_FENCE();
       double Dm[21];
 #pragma unroll
-      for (int m = 0; m < 21; ++m) Dm[m] = cpn[m] - csn[m];
-      double B1[6][6];  // G (C_p - C_s')
+      for (int m = 0; m < 21; ++m) Dm[m] = *rec(k + 1, a.off_pred + 6 + m) - csn[m];
 #pragma unroll
-      for (int r = 0; r < 6; ++r)
+      for (int r = 0; r < 6; ++r) {
+        double B1[6];  // row r of G (C_p - C_s')
 #pragma unroll
         for (int c = 0; c < 6; ++c) {
           double s = G[r][0] * Dm[sidx(0, c)];
 #pragma unroll
           for (int m = 1; m < 6; ++m) s = s + G[r][m] * Dm[sidx(m, c)];
-          B1[r][c] = s;
+          B1[c] = s;
         }
-#pragma unroll
-      for (int r = 0; r < 6; ++r)
 #pragma unroll
         for (int c = r; c < 6; ++c) {
-          double s = B1[r][0] * G[c][0];
+          double s = B1[0] * G[c][0];
 #pragma unroll
-          for (int m = 1; m < 6; ++m) s = s + B1[r][m] * G[c][m];
+          for (int m = 1; m < 6; ++m) s = s + B1[m] * G[c][m];
           csn[sidx(r, c)] = *rec(k, a.off_filt + 6 + sidx(r, c)) - s;
         }
+      }
       correctParams(xsn);
       if (a.full_record) storeBound(k, KF_TS_SMOOTHED, xsn, csn);
     }
@@ -913,31 +944,30 @@ struct TrackFit {
     return true;
   }
 
-  // Actor::finalize, KalmanFitter.hpp:360-393
-  KFD_INLINE bool finalize() {
-    smoothed = true;
-    double xs[6], cs[21];
-    if (!smooth(xs, cs)) return false;
-    // the first smoothed state sits on the first measured surface
-    updateFromBound(plane(firstPlane), xs, cs);
-    navDir = -1;
-    ss.init(-1.0 * fabs(a.max_step_size));
-    pathAcc = 0.0;
-    return true;
+  // Actor::operator(), KalmanFitter.hpp:216-273.  Forward kernel: filter and
+  // the finalisation check; returns true where finalize() must run (the
+  // backward kernel does it).  Backward kernel: only the post-finalisation
+  // part can act (smoothed is set).
+  KFD_INLINE bool actor() {
+    if constexpr (FWD) {
+      if (cur >= 0 && !smoothed && !finished) {
+        if (!filter(cur)) result = false;
+      }
+      if (finalizeCond()) {
+        if (!a.smoothing) {
+          finished = true;
+        } else {
+          return true;
+        }
+      }
+      return false;
+    } else {
+      actorPost();
+      return false;
+    }
   }
 
-  // Actor::operator(), KalmanFitter.hpp:216-273
-  KFD_INLINE void actor() {
-    if (cur >= 0 && !smoothed && !finished) {
-      if (!filter(cur)) result = false;
-    }
-    if (finalizeCond()) {
-      if (!a.smoothing) {
-        finished = true;
-      } else if (!finalize()) {
-        result = false;
-      }
-    }
+  KFD_INLINE void actorPost() {
     if (smoothed && !finished && surfaceReached()) {
       boundStateLine(fpar);
 #pragma unroll
@@ -946,121 +976,297 @@ struct TrackFit {
     }
   }
 
-  // ----------------------------------------------------------------- fit ----
-  KFD_INLINE void run() {
-    uint32_t status = 0;
-    h0 = a.hit_offset[i];
-    h1 = a.hit_offset[i + 1];
-    seqFirst = a.seq_first[i];
-    size = a.seq_count[i];
-    double pv[6];
-#pragma unroll
-    for (int k = 0; k < 6; ++k) pv[k] = a.start_params[(size_t)k * a.n + i];
-#pragma unroll
-    for (int k = 0; k < 21; ++k) cov[k] = a.start_cov[(size_t)k * a.n + i];
-#pragma unroll
-    for (int k = 0; k < 3; ++k) lc[k] = a.perigee[(size_t)k * a.n + i];
-#pragma unroll
-    for (int k = 0; k < 6; ++k) fpar[k] = 0.0;
-#pragma unroll
-    for (int k = 0; k < 21; ++k) fcov[k] = 0.0;
-    measured = 0;
-    firstPlane = 0;
-    smoothed = finished = smootherNaN = false;
-    result = true;
-    chi2sum = 0.0;
-    next = 0;
-    cur = -1;
-    targetReached = navBreak = false;
-    stepFailed = false;
-    bool terminated = false;
-    int steps = 0;
-    const bool bad = size <= 0 || seqFirst < 0 || seqFirst + size > a.n_planes || h0 < 0 ||
-                     h1 <= h0 || h1 > a.n_hits || pv[4] == 0.0 || a.max_states < 1;
-    if (bad) {
-      status |= KF_STATUS_BAD_INPUT;
-    } else {
-      start(pv);
-      navStatus();
-      actor();
-      if (!(kalmanAbort() || pathLimit())) {
-        navTarget();
-        while (steps < a.max_steps) {
-          bool event = false;
-          while (steps < a.max_steps) {
-            step();
-            navStatus();
-            if (cur >= 0 && !smoothed && !finished) {
-              event = true;
-              break;
-            }
-            if (finalizeCond()) {
-              event = true;
-              break;
-            }
-            if (smoothed && !finished && surfaceReached()) {
-              event = true;
-              break;
-            }
-            if (kalmanAbort() || pathLimit()) {
-              terminated = true;
-              break;
-            }
-            navTarget();
-            ++steps;
-          }
-          if (!event) break;
-          actor();
-          if (kalmanAbort() || pathLimit()) {
-            terminated = true;
-            break;
-          }
-          navTarget();
-          ++steps;
+  // The propagation loop (Propagator.ipp:39-69) with the cheap iterations in
+  // an inner loop.  Returns true if the forward kernel must hand over to the
+  // backward kernel (finalize due inside the actor).
+  KFD_INLINE bool loop(int &steps, bool &terminated) {
+    while (steps < a.max_steps) {
+      bool event = false;
+      while (steps < a.max_steps) {
+        step();
+        navStatus();
+        if (FWD && cur >= 0 && !smoothed && !finished) {
+          event = true;
+          break;
         }
+        if (FWD && finalizeCond()) {
+          event = true;
+          break;
+        }
+        if (!FWD && smoothed && !finished && surfaceReached()) {
+          event = true;
+          break;
+        }
+        if (kalmanAbort() || pathLimit()) {
+          terminated = true;
+          break;
+        }
+        navTarget();
+        ++steps;
       }
-      if (!terminated) navBreak = true;
-      actor();
-      if (result && measured == (int)(h1 - h0)) status |= KF_STATUS_OK;
-      if (smoothed) status |= KF_STATUS_SMOOTHED;
-      if (finished) status |= KF_STATUS_FINISHED;
-      if (!result) status |= KF_STATUS_RESULT_FAIL;
-      if (stepFailed) status |= KF_STATUS_STEP_FAIL;
-      if (!terminated && steps >= a.max_steps) status |= KF_STATUS_MAX_STEPS;
-      if (navBreak && measured != (int)(h1 - h0)) status |= KF_STATUS_NAV_BREAK;
-      if (smootherNaN) status |= KF_STATUS_SMOOTHER_NAN;
+      if (!event) break;
+      if (actor()) return true;
+      if (kalmanAbort() || pathLimit()) {
+        terminated = true;
+        break;
+      }
+      navTarget();
+      ++steps;
     }
+    return false;
+  }
+
+  // ------------------------------------------------------------- carry ----
+  // Per-track hand-over record from the forward to the backward kernel.
+  enum { kCI_resume = 0, kCI_steps, kCI_measured, kCI_first, kCI_next, kCI_flags, kCI_N };
+  enum { kCD_q = 0, kCD_J = 1, kCD_k4 = 26, kCD_dtds = 29, kCD_N = 30 };
+  KFD_INLINE int32_t &ci(int f) const { return a.carry_i[(size_t)f * a.n + i]; }
+  KFD_INLINE double &cd(int f) const { return a.carry_d[(size_t)f * a.n + i]; }
+
+  KFD_INLINE void saveCarry(int resume, int steps, bool terminated) const {
+    ci(kCI_resume) = resume;
+    ci(kCI_steps) = steps;
+    ci(kCI_measured) = measured;
+    ci(kCI_first) = firstPlane;
+    ci(kCI_next) = next;
+    ci(kCI_flags) = (result ? 1 : 0) | (stepFailed ? 2 : 0) | (navBreak ? 4 : 0) |
+                    (terminated ? 8 : 0) | (derivSet ? 16 : 0);
+    cd(kCD_q) = q;
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+      for (int c = 0; c < 5; ++c) cd(kCD_J + 5 * r + c) = (double)A[r][c];
+    cd(kCD_J + 15) = (double)b;
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) cd(kCD_J + 16 + 3 * r + c) = (double)G[r][c];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) cd(kCD_k4 + k) = (double)k4d[k];
+    cd(kCD_dtds) = (double)dtds;
+  }
+
+  KFD_INLINE int loadCarry(int &steps, bool &terminated) {
+    const int resume = ci(kCI_resume);
+    steps = ci(kCI_steps);
+    measured = ci(kCI_measured);
+    firstPlane = ci(kCI_first);
+    next = ci(kCI_next);
+    const int fl = ci(kCI_flags);
+    result = (fl & 1) != 0;
+    stepFailed = (fl & 2) != 0;
+    navBreak = (fl & 4) != 0;
+    terminated = (fl & 8) != 0;
+    derivSet = (fl & 16) != 0;
+    q = cd(kCD_q);
+    return resume;
+  }
+
+  KFD_INLINE void loadCarryJacobian() {
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+      for (int c = 0; c < 5; ++c) A[r][c] = (S)cd(kCD_J + 5 * r + c);
+    b = (S)cd(kCD_J + 15);
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) G[r][c] = (S)cd(kCD_J + 16 + 3 * r + c);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) k4d[k] = (S)cd(kCD_k4 + k);
+    dtds = (S)cd(kCD_dtds);
+  }
+
+  // ----------------------------------------------------------------- fit ----
+  KFD_INLINE uint32_t statusBits(int steps, bool terminated) const {
+    uint32_t status = 0;
+    if (result && measured == (int)(h1 - h0)) status |= KF_STATUS_OK;
+    if (smoothed) status |= KF_STATUS_SMOOTHED;
+    if (finished) status |= KF_STATUS_FINISHED;
+    if (!result) status |= KF_STATUS_RESULT_FAIL;
+    if (stepFailed) status |= KF_STATUS_STEP_FAIL;
+    if (!terminated && steps >= a.max_steps) status |= KF_STATUS_MAX_STEPS;
+    if (navBreak && measured != (int)(h1 - h0)) status |= KF_STATUS_NAV_BREAK;
+    if (smootherNaN) status |= KF_STATUS_SMOOTHER_NAN;
+    return status;
+  }
+
+  KFD_INLINE void writeFinal(uint32_t status) const {
     if (a.out_params)
 #pragma unroll
       for (int k = 0; k < 6; ++k) a.out_params[(size_t)k * a.n + i] = fpar[k];
     if (a.out_cov)
 #pragma unroll
       for (int k = 0; k < 21; ++k) a.out_cov[(size_t)k * a.n + i] = fcov[k];
+    if (a.out_status) a.out_status[i] = status;
+  }
+
+  KFD_INLINE void loadTrack() {
+    h0 = a.hit_offset[i];
+    h1 = a.hit_offset[i + 1];
+    seqFirst = a.seq_first[i];
+    size = a.seq_count[i];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) lc[k] = a.perigee[(size_t)k * a.n + i];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) fpar[k] = 0.0;
+#pragma unroll
+    for (int k = 0; k < 21; ++k) fcov[k] = 0.0;
+    smoothed = finished = smootherNaN = false;
+    cur = -1;
+    targetReached = false;
+  }
+
+  // Forward kernel body: Propagator::propagate (Propagator.ipp:3-96) up to
+  // the finalisation.
+  KFD_INLINE void runForward() {
+    loadTrack();
+    double pv[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) pv[k] = a.start_params[(size_t)k * a.n + i];
+#pragma unroll
+    for (int k = 0; k < 21; ++k) cov[k] = a.start_cov[(size_t)k * a.n + i];
+    measured = 0;
+    firstPlane = 0;
+    result = true;
+    chi2sum = 0.0;
+    next = 0;
+    navBreak = false;
+    stepFailed = false;
+    bool terminated = false;
+    int steps = 0;
+    int resume = kResumeNone;
+    const bool bad = size <= 0 || seqFirst < 0 || seqFirst + size > a.n_planes || h0 < 0 ||
+                     h1 <= h0 || h1 > a.n_hits || pv[4] == 0.0 || a.max_states < 1;
+    if (!bad) {
+      start(pv);
+      navStatus();
+      if (actor()) {
+        resume = kResumePreLoop;
+      } else {
+        if (!(kalmanAbort() || pathLimit())) {
+          navTarget();
+          if (loop(steps, terminated)) resume = kResumeInLoop;
+        }
+        if (resume == kResumeNone) {
+          if (!terminated) navBreak = true;
+          if (actor()) resume = kResumePostLoop;
+        }
+      }
+    }
     if (a.out_chi2) a.out_chi2[i] = chi2sum;
     if (a.out_ndf) a.out_ndf[i] = 2 * measured;
     if (a.out_nmeas) a.out_nmeas[i] = measured;
-    if (a.out_status) a.out_status[i] = status;
+    if (resume != kResumeNone) {
+      saveCarry(resume, steps, terminated);
+    } else {
+      ci(kCI_resume) = kResumeNone;
+      writeFinal(bad ? KF_STATUS_BAD_INPUT : statusBits(steps, terminated));
+    }
+  }
+
+  // Backward kernel body: Actor::finalize (KalmanFitter.hpp:360-393), the
+  // rest of that actor call, and the propagation back to the perigee.
+  KFD_INLINE void runBackward() {
+    int steps = 0;
+    bool terminated = false;
+    loadTrack();
+    const int resume = loadCarry(steps, terminated);
+    if (resume == kResumeNone) return;
+    // finalize
+    smoothed = true;
+    double xs[6], cs[21];
+    if (smooth(xs, cs)) {
+      updateFromBound(plane(firstPlane), xs, cs);
+      navDir = -1;
+      ss.init(-1.0 * fabs(a.max_step_size));
+      pathAcc = 0.0;
+    } else {
+      result = false;
+    }
+    loadCarryJacobian();
+    actorPost();
+    if (resume == kResumePreLoop) {
+      if (!(kalmanAbort() || pathLimit())) {
+        navTarget();
+        loop(steps, terminated);
+      }
+      if (!terminated) navBreak = true;
+      actor();
+    } else if (resume == kResumeInLoop) {
+      if (kalmanAbort() || pathLimit()) {
+        terminated = true;
+      } else {
+        navTarget();
+        ++steps;
+        loop(steps, terminated);
+      }
+      if (!terminated) navBreak = true;
+      actor();
+    }
+    writeFinal(statusBits(steps, terminated));
   }
 };
 
+// Base of the state store of the 64-track chunk this wave works on.  Blocks
+// are multiples of 64 work-items, so a wave covers exactly one chunk.
+KFD_INLINE double *chunkBase(const FitArgs &a, int64_t i) {
+  const uint32_t chunk = __builtin_amdgcn_readfirstlane((uint32_t)(i >> 6));
+  return a.ts + (size_t)chunk * (size_t)a.max_states * (size_t)a.ts_fields * 64u;
+}
+
 template <typename S>
-__global__ void __launch_bounds__(KF_BLOCK) kf_fit_kernel(FitArgs args) {
+__global__ void __launch_bounds__(KF_BLOCK) kf_forward_kernel(FitArgs args) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  double *base = chunkBase(args, i);
+  if (i >= args.n) return;
+  TrackFit<S, true> tf(args, i, base, (uint32_t)(i & 63));
+  tf.runForward();
+}
+
+template <typename S>
+__global__ void __launch_bounds__(KF_BLOCK) kf_backward_kernel(FitArgs args) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  double *base = chunkBase(args, i);
+  if (i >= args.n) return;
+  TrackFit<S, false> tf(args, i, base, (uint32_t)(i & 63));
+  tf.runBackward();
+}
+
+// Copies the state store into kf_results.track_states ([state][120][n]).
+__global__ void __launch_bounds__(256) kf_export_states(FitArgs args, double *out) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= args.n) return;
-  TrackFit<S> tf(args, i);
-  tf.run();
+  const int nm = args.out_nmeas[i];
+  const double *base = args.ts + (size_t)(i >> 6) * (size_t)args.max_states * KF_TS_STRIDE * 64u;
+  const uint32_t lane = (uint32_t)(i & 63);
+  for (int k = 0; k < nm && k < args.max_states; ++k)
+    for (int f = 0; f < KF_TS_STRIDE; ++f)
+      out[((size_t)k * KF_TS_STRIDE + f) * (size_t)args.n + i] =
+          base[((uint32_t)k * KF_TS_STRIDE + (uint32_t)f) * 64u + lane];
 }
 
 }  // namespace kfd
 
-extern "C" hipError_t kfd_launch_fit(const kfd::FitArgs *args, int precision, hipStream_t stream) {
+extern "C" hipError_t kfd_launch_fit(const kfd::FitArgs *args, int precision, hipStream_t stream,
+                                     double *track_states_out) {
   const int64_t blocks = (args->n + KF_BLOCK - 1) / KF_BLOCK;
   if (blocks <= 0) return hipSuccess;
-  if (precision == KF_PRECISION_FP64)
-    hipLaunchKernelGGL(kfd::kf_fit_kernel<double>, dim3((unsigned)blocks), dim3(KF_BLOCK), 0, stream,
-                       *args);
-  else
-    hipLaunchKernelGGL(kfd::kf_fit_kernel<float>, dim3((unsigned)blocks), dim3(KF_BLOCK), 0, stream,
-                       *args);
+  if (precision == KF_PRECISION_FP64) {
+    hipLaunchKernelGGL(kfd::kf_forward_kernel<double>, dim3((unsigned)blocks), dim3(KF_BLOCK), 0,
+                       stream, *args);
+    hipLaunchKernelGGL(kfd::kf_backward_kernel<double>, dim3((unsigned)blocks), dim3(KF_BLOCK), 0,
+                       stream, *args);
+  } else {
+    hipLaunchKernelGGL(kfd::kf_forward_kernel<float>, dim3((unsigned)blocks), dim3(KF_BLOCK), 0,
+                       stream, *args);
+    hipLaunchKernelGGL(kfd::kf_backward_kernel<float>, dim3((unsigned)blocks), dim3(KF_BLOCK), 0,
+                       stream, *args);
+  }
+  if (track_states_out) {
+    const int64_t eb = (args->n + 255) / 256;
+    hipLaunchKernelGGL(kfd::kf_export_states, dim3((unsigned)eb), dim3(256), 0, stream, *args,
+                       track_states_out);
+  }
   return hipGetLastError();
 }

@@ -7,7 +7,7 @@
 
 #include "kf_device.hpp"
 
-#define KF_BLOCK 128
+#define KF_BLOCK 128  // multiple of 64: a wave covers one 64-track store chunk
 
 namespace kfd {
 
@@ -41,14 +41,18 @@ struct FitArgs {
   int32_t *out_ndf;
   int32_t *out_nmeas;
   uint32_t *out_status;
-  // per-state store: element f of state k of track i at
-  // ts[(k * ts_fields + f) * n + i]
+  // per-state store, chunked by 64 tracks: element f of state k of track i at
+  // ts[((i / 64) * max_states * ts_fields + k * ts_fields + f) * 64 + i % 64]
   double *ts;
   int32_t ts_fields;
   int32_t off_pred, off_filt, off_jac;
-  int32_t full_record;  // 1: ts is kf_results.track_states (KF_TS_* layout)
+  int32_t full_record;  // 1: fields use the KF_TS_* layout (120), exported after the fit
+  // forward -> backward kernel hand-over, [field][n]
+  int32_t *carry_i;  // 6 int fields
+  double *carry_d;   // 30 double fields
 };
 
 }  // namespace kfd
 
-extern "C" hipError_t kfd_launch_fit(const kfd::FitArgs *args, int precision, hipStream_t stream);
+extern "C" hipError_t kfd_launch_fit(const kfd::FitArgs *args, int precision, hipStream_t stream,
+                                     double *track_states_out);
