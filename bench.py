@@ -9,12 +9,17 @@ whole batch with inputs already resident in HBM; the timed region is exactly
 --steps fits bracketed by device synchronisation.  Fits run in every precision
 given by --precision (first one is the headline `value`).
 
-The roofline object prices the fit kernel against MI355X's fp64 vector peak
-(78.6 TFLOP/s, equal to its fp64 matrix peak) with the algorithmic flop count
-per track of the reference formulation (structural zeros and identities
-removed), measured by the counting build of the CPU oracle on this workload:
-FLOPS_PER_TRACK below.  The kernel duration is the mean over the timed steps of
-HIP events recorded by the library on the launch stream.
+A fit is two launches on one stream: the forward filter (start, steps,
+gain-matrix update on every surface) and the smoother plus back-propagation
+to the perigee.  The roofline object prices the dominant one of the headline
+precision against MI355X's vector peaks, fp64 78.6 TFLOP/s and fp32
+157.3 TFLOP/s, with T_roof = F32 / P32 + F64 / P64 per track: F is the
+algorithmic flop count of the reference formulation (structural zeros and
+identities removed) of that launch's share of the fit, F32 the part the mixed
+mode runs in float (0 in fp64 mode), both measured by the counting build of
+the CPU oracle on this workload (constants below).  Launch durations are the
+mean over the timed steps of HIP events the library records on the launch
+stream before, between and after the two launches.
 """
 import argparse
 import json
@@ -27,13 +32,20 @@ sys.path.insert(0, ROOT)
 
 METRIC = "fitted tracks/sec on 1 GPU: FP32-mixed vs FP64 at fixed error; adjoint trk/s"
 # Algorithmic flops per config-2 track (mean over 20,000 tracks of the counting
-# oracle, liboracle_count.so; tests/test_bench_constants.py re-measures it).
+# oracle, liboracle_count.so; tests/test_bench_constants.py re-measures them):
+# whole fit, the smoother/back-propagation launch's share, and the float-class
+# parts of the mixed mode.
 FLOPS_PER_TRACK = 30610.0
+FLOPS_BWD_PER_TRACK = 17876.0
+FLOPS32_PER_TRACK = 8109.0
+FLOPS32_BWD_PER_TRACK = 784.0
 FP64_PEAK_TFLOPS = 78.6
+FP32_PEAK_TFLOPS = 157.3
 HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch from rocprofv3 FETCH_SIZE + WRITE_SIZE (scaled by the
 # copy-kernel calibration), per precision; None until profiled.
 TRAFFIC_BYTES_PER_LAUNCH = {}
+PARAM_NAMES = ("loc0", "loc1", "phi", "theta", "qop", "t")
 
 
 def parse():
@@ -43,7 +55,8 @@ def parse():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--config", type=int, default=2)
     ap.add_argument("--tracks", type=int, default=None, help="override the workload size")
-    ap.add_argument("--precision", default="fp64", help="comma list of fp64,mixed")
+    ap.add_argument("--precision", default="mixed,fp64",
+                    help="comma list of mixed,fp64 (the first is the headline value)")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = all host cores")
     ap.add_argument("--cpu-sample", type=int, default=1048576,
                     help="tracks of the same workload timed on the CPU oracle")
@@ -66,21 +79,65 @@ def copy_calibration(mib):
     print(json.dumps({"copy_calibration_bytes": n * 4}), flush=True)
 
 
-def cpu_baseline(batch, opts_c, threads, sample):
+def cpu_baseline(batch, opts_c, threads, sample, gpu_results):
     """The CPU oracle (FP64 restatement of the reference fit, OpenMP over
-    tracks as Run/CPU/KalmanFitterCPUTest.cpp) on a bounded sample."""
+    tracks as Run/CPU/KalmanFitterCPUTest.cpp) timed on a bounded sample.
+    Its results on that sample also give the per-parameter error of each GPU
+    precision mode (|x_gpu - x_cpu| / max(|x_cpu|, sigma_cpu), max and 99.9 %
+    quantile over the fitted tracks)."""
     import numpy as np
     import oracle
+    from gpukalmanfitter_amd import abi
     n = min(sample, batch.n_tracks)
     sub = batch.subset(np.arange(n))
     if threads <= 0:
         threads = os.cpu_count() or 1
     oracle.time_fit(sub.subset(np.arange(min(n, 2000))), opts_c, threads)  # warm
     sec = oracle.time_fit(sub, opts_c, threads)
-    return {"value": n / sec, "unit": "tracks/s", "cores": threads, "kind": "port",
+    base = {"value": n / sec, "unit": "tracks/s", "cores": threads, "kind": "port",
             "sample": f"first {n} tracks of the same workload, FP64 oracle "
                       f"(oracle/kfo_fitter.hpp), OpenMP over tracks, {threads} threads, "
                       f"fit loop timed alone ({sec:.2f} s)"}
+    ores, _, _, _ = oracle.fit(sub, opts_c, n_threads=threads)
+    ok = (ores.status & abi.KF_STATUS_OK) != 0
+    sig = np.sqrt(np.abs(ores.cov[abi.DIAG_INDEX][:, ok]))
+    scale = np.maximum(np.abs(ores.params[:, ok]), sig)
+    errors = {}
+    for prec, r in gpu_results.items():
+        dev = np.abs(r["params"][:, :n][:, ok] - ores.params[:, ok]) / scale
+        errors[prec] = {
+            "status_equal": bool(np.array_equal(r["status"][:n], ores.status)),
+            "max": {nm: float(dev[k].max()) for k, nm in enumerate(PARAM_NAMES)},
+            "q999": {nm: float(np.quantile(dev[k], 0.999)) for k, nm in enumerate(PARAM_NAMES)},
+        }
+    return base, errors
+
+
+def roofline(prec, n, phase_ms):
+    """Roofline of the dominant launch (see the module docstring)."""
+    mixed = prec == "mixed"
+    flops = {"forward": FLOPS_PER_TRACK - FLOPS_BWD_PER_TRACK, "backward": FLOPS_BWD_PER_TRACK}
+    f32 = {"forward": (FLOPS32_PER_TRACK - FLOPS32_BWD_PER_TRACK) if mixed else 0.0,
+           "backward": FLOPS32_BWD_PER_TRACK if mixed else 0.0}
+    ms = {"forward": phase_ms[0], "backward": phase_ms[1]}
+    k = max(ms, key=ms.get)
+    f, g = flops[k], f32[k]
+    t_roof_per_track = g / (FP32_PEAK_TFLOPS * 1e12) + (f - g) / (FP64_PEAK_TFLOPS * 1e12)
+    peak = f / t_roof_per_track / 1e12
+    achieved = f * n / (ms[k] * 1e-3) / 1e12
+    other = "backward" if k == "forward" else "forward"
+    return {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
+            "frac": achieved / peak, "traffic": TRAFFIC_BYTES_PER_LAUNCH.get((prec, k)),
+            "kernel": {"forward": "kf_forward_kernel", "backward": "kf_backward_kernel"}[k]
+                      + ("<float>" if mixed else "<double>"),
+            "kernel_ms": ms[k], "flops_per_track": f, "fp32_flops_per_track": g,
+            "compute": "vector ALU, not MFMA: peak = F / (F32/157.3 + F64/78.6 TFLOP/s), the "
+                       "MI355X fp32/fp64 vector peaks weighted by this launch's flop mix",
+            "other_launch": {"kernel_ms": ms[other], "flops_per_track": flops[other],
+                             "fp32_flops_per_track": f32[other],
+                             "frac": flops[other] * n / (ms[other] * 1e-3) / 1e12 /
+                             (flops[other] / (f32[other] / FP32_PEAK_TFLOPS +
+                                              (flops[other] - f32[other]) / FP64_PEAK_TFLOPS))}}
 
 
 def main():
@@ -116,43 +173,42 @@ def main():
     precisions = [p.strip() for p in args.precision.split(",") if p.strip()]
     rates = {}
     kernel_ms = {}
+    phase_ms = {}
     ms_step = {}
+    results = {}
     for prec in precisions:
         opts = KalmanFitterOptions(precision=prec, **configs.WORKLOAD_FIT_OPTIONS)
         for _ in range(args.warmup):
             kf.fit_batch(dev, opts)
         torch.cuda.synchronize()
         kms = []
+        pms = []
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         for _ in range(args.steps):
             kf.fit_batch(dev, opts)
             kms.append(kf.last_kernel_ms())
+            pms.append(kf.last_phase_ms())
         torch.cuda.synchronize()
         el = time.perf_counter() - t0
         rates[prec] = n * args.steps / el
         ms_step[prec] = el * 1e3 / args.steps
         kernel_ms[prec] = float(np.mean(kms))
-        st = dev.results_numpy()["status"]
+        phase_ms[prec] = [float(x) for x in np.mean(np.array(pms), axis=0)]
+        results[prec] = dev.results_numpy()
+        st = results[prec]["status"]
         ok = float(((st & abi.KF_STATUS_OK) != 0).mean())
         print(f"[bench] {prec}: {rates[prec]:.4g} tracks/s, {ms_step[prec]:.3f} ms/step, "
-              f"kernel {kernel_ms[prec]:.3f} ms, fitted {ok:.4f}", file=sys.stderr, flush=True)
+              f"kernels {kernel_ms[prec]:.3f} ms (forward {phase_ms[prec][0]:.3f}, "
+              f"backward {phase_ms[prec][1]:.3f}), fitted {ok:.4f}", file=sys.stderr, flush=True)
 
     head = precisions[0]
-    roof = None
-    if FLOPS_PER_TRACK:
-        achieved = FLOPS_PER_TRACK * n / (kernel_ms[head] * 1e-3) / 1e12
-        roof = {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
-                "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
-                "traffic": TRAFFIC_BYTES_PER_LAUNCH.get(head),
-                "compute": "fp64 vector ALU (MI355X fp64 vector peak equals its fp64 "
-                           "matrix peak, 78.6 TFLOP/s); algorithmic flops/track "
-                           f"{FLOPS_PER_TRACK:.0f} from the counting oracle",
-                "kernel_ms": kernel_ms[head]}
+    roof = roofline(head, n, phase_ms[head]) if args.config == 2 else None
     cpu = None
+    errors = None
     if not args.no_cpu:
-        cpu = cpu_baseline(batch, abi.default_options(**configs.WORKLOAD_FIT_OPTIONS),
-                           args.cpu_threads, args.cpu_sample)
+        cpu, errors = cpu_baseline(batch, abi.default_options(**configs.WORKLOAD_FIT_OPTIONS),
+                                   args.cpu_threads, args.cpu_sample, results)
         print(f"[bench] cpu baseline {cpu['value']:.4g} tracks/s on {cpu['cores']} threads",
               file=sys.stderr, flush=True)
     out = {
@@ -177,6 +233,7 @@ def main():
         "h2d_s": h2d_s,
         "roofline": roof,
         "cpu_baseline": cpu,
+        "error_vs_oracle": errors,
     }
     if "fp64" in rates and "mixed" in rates:
         out["mixed_over_fp64"] = rates["mixed"] / rates["fp64"]

@@ -25,11 +25,11 @@ struct kf_ctx {
   bool field_set = false;
   float *grid = nullptr;
   size_t grid_elems = 0;
-  double *store = nullptr;  // per-state store
+  void *store = nullptr;  // per-state stores of the smoother (and track-state records)
   size_t store_bytes = 0;
   void *carry = nullptr;    // forward -> backward kernel hand-over
   size_t carry_bytes = 0;
-  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  hipEvent_t ev0 = nullptr, evm = nullptr, ev1 = nullptr;
   bool timed = false;
 };
 
@@ -46,12 +46,33 @@ int fail(kf_ctx *ctx, int code, const char *fmt, ...) {
   return code;
 }
 
+// Grows a device buffer to at least `need` bytes (contents are not kept).
+int grow(kf_ctx *ctx, void **buf, size_t *cap, size_t need, hipStream_t s, const char *what);
+
 #define KF_HIP(ctx, call)                                                            \
   do {                                                                               \
     hipError_t e_ = (call);                                                          \
     if (e_ != hipSuccess)                                                            \
       return fail((ctx), KF_E_HIP, "%s failed: %s", #call, hipGetErrorString(e_));   \
   } while (0)
+
+int grow(kf_ctx *ctx, void **buf, size_t *cap, size_t need, hipStream_t s, const char *what) {
+  if (need <= *cap) return KF_OK;
+  if (*buf) {
+    KF_HIP(ctx, hipStreamSynchronize(s));
+    KF_HIP(ctx, hipFree(*buf));
+    *buf = nullptr;
+    *cap = 0;
+  }
+  hipError_t e = hipMalloc(buf, need);
+  if (e != hipSuccess) {
+    *buf = nullptr;
+    return fail(ctx, KF_E_NOMEM, "kf_fit: cannot allocate %zu bytes of %s: %s", need, what,
+                hipGetErrorString(e));
+  }
+  *cap = need;
+  return KF_OK;
+}
 
 // Physics constants evaluated the way the reference's unit literals do
 // (long double literal times a double unit, rounded to double;
@@ -107,6 +128,7 @@ int kf_ctx_create(int device, kf_ctx **out) {
   e = hipSetDevice(device);
   if (e == hipSuccess) e = hipEventCreate(&ctx->ev0);
   if (e == hipSuccess) e = hipEventCreate(&ctx->ev1);
+  if (e == hipSuccess) e = hipEventCreate(&ctx->evm);
   if (e != hipSuccess) {
     fail(nullptr, KF_E_HIP, "kf_ctx_create: %s", hipGetErrorString(e));
     delete ctx;
@@ -126,6 +148,7 @@ int kf_ctx_destroy(kf_ctx *ctx) {
   if (ctx->carry) (void)hipFree(ctx->carry);
   if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
   if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+  if (ctx->evm) (void)hipEventDestroy(ctx->evm);
   delete ctx;
   return KF_OK;
 }
@@ -281,61 +304,34 @@ int kf_fit(kf_ctx *ctx, const kf_tracks *tr, const kf_options *opt, const kf_res
   a.out_status = res->status;
 
   KF_HIP(ctx, hipSetDevice(ctx->device));
-  {
-    // state store, chunked by 64 tracks (see kf_kernels.hpp)
-    const int fields = res->track_states ? KF_TS_STRIDE : (27 + 27 + 36);
-    const size_t chunks = ((size_t)tr->n_tracks + 63) / 64;
-    const size_t need = chunks * 64 * (size_t)fields * (size_t)max_states * sizeof(double);
-    if (need > ctx->store_bytes) {
-      if (ctx->store) {
-        KF_HIP(ctx, hipStreamSynchronize((hipStream_t)stream));
-        KF_HIP(ctx, hipFree(ctx->store));
-        ctx->store = nullptr;
-        ctx->store_bytes = 0;
-      }
-      hipError_t e = hipMalloc(&ctx->store, need);
-      if (e != hipSuccess)
-        return fail(ctx, KF_E_NOMEM, "kf_fit: cannot allocate %zu bytes of state store: %s", need,
-                    hipGetErrorString(e));
-      ctx->store_bytes = need;
-    }
-    a.ts = ctx->store;
-    a.ts_fields = fields;
-    if (res->track_states) {
-      a.off_pred = KF_TS_PREDICTED;
-      a.off_filt = KF_TS_FILTERED;
-      a.off_jac = KF_TS_JACOBIAN;
-      a.full_record = 1;
-    } else {
-      a.off_pred = 0;
-      a.off_filt = 27;
-      a.off_jac = 54;
-      a.full_record = 0;
-    }
-  }
   if (res->track_states && !res->n_measured)
     return fail(ctx, KF_E_INVALID, "kf_fit: track_states needs results.n_measured");
+  hipStream_t s = (hipStream_t)stream;
   {
-    const size_t need = (size_t)tr->n_tracks * (6 * sizeof(int32_t) + 30 * sizeof(double));
-    if (need > ctx->carry_bytes) {
-      if (ctx->carry) {
-        KF_HIP(ctx, hipStreamSynchronize((hipStream_t)stream));
-        KF_HIP(ctx, hipFree(ctx->carry));
-        ctx->carry = nullptr;
-        ctx->carry_bytes = 0;
-      }
-      hipError_t e = hipMalloc(&ctx->carry, need);
-      if (e != hipSuccess)
-        return fail(ctx, KF_E_NOMEM, "kf_fit: cannot allocate %zu bytes of hand-over buffer: %s",
-                    need, hipGetErrorString(e));
-      ctx->carry_bytes = need;
-    }
+    // per-state stores, chunked by 64 tracks (layout in kf_kernels.hpp)
+    const size_t slots = ((size_t)tr->n_tracks + 63) / 64 * 64 * (size_t)max_states;
+    const size_t s_size = (opt->precision == KF_PRECISION_FP64) ? sizeof(double) : sizeof(float);
+    const size_t d_bytes = slots * KF_SD_N * sizeof(double);
+    const size_t s_bytes = (slots * KF_SS_N * s_size + 255) / 256 * 256;
+    const size_t f_bytes = res->track_states ? slots * KF_TS_STRIDE * sizeof(double) : 0;
+    int rc = grow(ctx, &ctx->store, &ctx->store_bytes, d_bytes + s_bytes + f_bytes, s,
+                  "state store");
+    if (rc != KF_OK) return rc;
+    char *base = (char *)ctx->store;
+    a.tss = base;
+    a.tsd = (double *)(base + s_bytes);
+    a.tsf = res->track_states ? (double *)(base + s_bytes + d_bytes) : nullptr;
+  }
+  {
+    int rc = grow(ctx, &ctx->carry, &ctx->carry_bytes,
+                  (size_t)tr->n_tracks * (6 * sizeof(int32_t) + 30 * sizeof(double)), s,
+                  "hand-over buffer");
+    if (rc != KF_OK) return rc;
     a.carry_d = (double *)ctx->carry;
     a.carry_i = (int32_t *)((char *)ctx->carry + (size_t)tr->n_tracks * 30 * sizeof(double));
   }
-  hipStream_t s = (hipStream_t)stream;
   KF_HIP(ctx, hipEventRecord(ctx->ev0, s));
-  hipError_t e = kfd_launch_fit(&a, opt->precision, s, res->track_states);
+  hipError_t e = kfd_launch_fit(&a, opt->precision, s, res->track_states, ctx->evm);
   if (e != hipSuccess) return fail(ctx, KF_E_HIP, "kf_fit: launch failed: %s", hipGetErrorString(e));
   KF_HIP(ctx, hipEventRecord(ctx->ev1, s));
   ctx->timed = true;
@@ -348,6 +344,20 @@ double kf_last_kernel_ms(const kf_ctx *ctx) {
   float ms = 0.0f;
   if (hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1) != hipSuccess) return -1.0;
   return (double)ms;
+}
+
+int kf_last_phase_ms(const kf_ctx *ctx, double *forward_ms, double *backward_ms) {
+  if (!ctx || !forward_ms || !backward_ms) return KF_E_INVALID;
+  *forward_ms = *backward_ms = -1.0;
+  if (!ctx->timed) return KF_E_STATE;
+  if (hipEventSynchronize(ctx->ev1) != hipSuccess) return KF_E_HIP;
+  float f = 0.0f, b = 0.0f;
+  if (hipEventElapsedTime(&f, ctx->ev0, ctx->evm) != hipSuccess ||
+      hipEventElapsedTime(&b, ctx->evm, ctx->ev1) != hipSuccess)
+    return KF_E_HIP;
+  *forward_ms = (double)f;
+  *backward_ms = (double)b;
+  return KF_OK;
 }
 
 }  // extern "C"

@@ -23,6 +23,7 @@ enum { kResumeNone = 0, kResumePreLoop = 1, kResumeInLoop = 2, kResumePostLoop =
 
 template <typename S, bool FWD>
 struct TrackFit {
+  static constexpr bool kMixed = sizeof(S) < sizeof(double);
   const FitArgs &a;
   const int64_t i;
 
@@ -60,32 +61,35 @@ struct TrackFit {
   double fpar[6];
   double fcov[21];
 
-  // per-state store of this wave's 64-track chunk (wave-uniform base) and
-  // this lane's slot in it
-  double *tsb;
+  // per-state stores of this wave's 64-track chunk (wave-uniform bases) and
+  // this lane's slot in them
+  double *tsd;
+  S *tss;
+  double *tsf;
   uint32_t lane;
 
-  KFD_INLINE TrackFit(const FitArgs &args, int64_t idx, double *chunk_base, uint32_t ln)
-      : a(args), i(idx), tsb(chunk_base), lane(ln) {}
+  KFD_INLINE TrackFit(const FitArgs &args, int64_t idx, double *d_base, S *s_base, double *f_base,
+                      uint32_t ln)
+      : a(args), i(idx), tsd(d_base), tss(s_base), tsf(f_base), lane(ln) {}
 
   // ------------------------------------------------------------ storage ----
-  // Element f of state k: tsb[(k * F + f) * 64 + lane] -- a 32-bit lane
+  // Element f of state k: base[(k * F + f) * 64 + lane] -- a 32-bit lane
   // offset from a wave-uniform base, so each access is one saddr load/store.
-  KFD_INLINE double *rec(int k, int f) const {
-    return tsb + (uint32_t)(((uint32_t)k * (uint32_t)a.ts_fields + (uint32_t)f) * 64u + lane);
+  KFD_INLINE static uint32_t slot(int k, int F, int f, uint32_t ln) {
+    return (uint32_t)(((uint32_t)k * (uint32_t)F + (uint32_t)f) * 64u + ln);
   }
-  KFD_INLINE void storeBound(int k, int off, const double par[6], const double c[21]) const {
+  KFD_INLINE double *recD(int k, int f) const { return tsd + slot(k, KF_SD_N, f, lane); }
+  KFD_INLINE S *recS(int k, int f) const { return tss + slot(k, KF_SS_N, f, lane); }
+  KFD_INLINE double *recF(int k, int f) const { return tsf + slot(k, KF_TS_STRIDE, f, lane); }
+  // public record: parameters + covariance at offset off
+  KFD_INLINE void storeBoundF(int k, int off, const double par[6], const double c[21]) const {
 #pragma unroll
-    for (int m = 0; m < 6; ++m) *rec(k, off + m) = par[m];
+    for (int m = 0; m < 6; ++m) *recF(k, off + m) = par[m];
 #pragma unroll
-    for (int m = 0; m < 21; ++m) *rec(k, off + 6 + m) = c[m];
+    for (int m = 0; m < 21; ++m) *recF(k, off + 6 + m) = c[m];
   }
-  KFD_INLINE void loadBound(int k, int off, double par[6], double c[21]) const {
-#pragma unroll
-    for (int m = 0; m < 6; ++m) par[m] = *rec(k, off + m);
-#pragma unroll
-    for (int m = 0; m < 21; ++m) c[m] = *rec(k, off + 6 + m);
-  }
+  // packed index of Jacobian entry (r, c), r in {0, 1, 2, 3, 5}, c < 5
+  KFD_INLINE static constexpr int jidx(int r, int c) { return ((r < 4) ? r : 4) * 5 + c; }
 
   // -------------------------------------------------------------- field ----
   KFD_INLINE void field(const double x[3], double B[3]) const { fieldAt(a.field, x[0], x[1], x[2], B); }
@@ -405,41 +409,47 @@ struct TrackFit {
       jb[4][c] = jp[7];
       jb[5][c] = jp[3];
     }
-    // cov <- jb cov jb^T.  Row 4 of jb is e4 and column 5 is e5 exactly (the
-    // q/p row and the time column of the transport), so those terms are
-    // skipped; row r of jb cov is formed and consumed at once.
-    double cn[21];
+    S cs[21];
+#pragma unroll
+    for (int m = 0; m < 21; ++m) cs[m] = (S)cov[m];
+    sandwich(jb, cs, cov);
+  }
+
+  // cn = jb c jb^T in S.  Row 4 of jb is e4 and column 5 is e5 exactly (the
+  // q/p row and the time column of the transport), so those terms are
+  // skipped; row r of jb c is formed and consumed at once.  The smoother
+  // calls this with the stored operands to recompute the predicted
+  // covariance bit for bit.
+  KFD_INLINE static void sandwich(const S jb[6][6], const S c[21], double cn[21]) {
 #pragma unroll
     for (int r = 0; r < 6; ++r) {
       S T[6];
 #pragma unroll
       for (int k = 0; k < 6; ++k) {
         if (r == 4) {
-          T[k] = (S)cov[sidx(4, k)];
+          T[k] = c[sidx(4, k)];
         } else {
-          S s = jb[r][0] * (S)cov[sidx(0, k)];
+          S s = jb[r][0] * c[sidx(0, k)];
 #pragma unroll
-          for (int c = 1; c < 5; ++c) s = s + jb[r][c] * (S)cov[sidx(c, k)];
-          if (r == 5) s = s + (S)cov[sidx(5, k)];
+          for (int m = 1; m < 5; ++m) s = s + jb[r][m] * c[sidx(m, k)];
+          if (r == 5) s = s + c[sidx(5, k)];
           T[k] = s;
         }
       }
 #pragma unroll
-      for (int c = r; c < 6; ++c) {
+      for (int cc = r; cc < 6; ++cc) {
         S s;
-        if (c == 4) {
+        if (cc == 4) {
           s = T[4];
         } else {
-          s = T[0] * jb[c][0];
+          s = T[0] * jb[cc][0];
 #pragma unroll
-          for (int k = 1; k < 5; ++k) s = s + T[k] * jb[c][k];
-          if (c == 5) s = s + T[5];
+          for (int m = 1; m < 5; ++m) s = s + T[m] * jb[cc][m];
+          if (cc == 5) s = s + T[5];
         }
-        cn[sidx(r, c)] = (double)s;
+        cn[sidx(r, cc)] = (double)s;
       }
     }
-#pragma unroll
-    for (int m = 0; m < 21; ++m) cov[m] = cn[m];
   }
 
   // boundState on a plane (EigenStepper.ipp:516-540, CovarianceEngine.hpp:
@@ -653,72 +663,83 @@ struct TrackFit {
   // PointwiseMaterialInteraction (.hpp:66-148, .ipp:14-53) and the RPP
   // formulas of Interactions.ipp (Bethe :150-176, Landau sigma :316-342,
   // Highland :501-546).  Only plane surfaces carry material (full update).
+  // The formulas run in S: in the mixed mode the energy loss (~1e-4 of p)
+  // and the noise variances need no more than float's 1e-7 relative; the
+  // momentum itself is updated in double.
   KFD_INLINE void material(const DevPlane &pl) {
     if (!pl.has_material) return;
     const double momentum = p;
-    const double qq = q;
-    const double qOverP = qq / momentum;
-    const double mass = a.mass;
-    const double nr[3] = {pl.r[2], pl.r[5], pl.r[8]};
-    const double pcor = 1.0 / fabs(dot3(nr, dir));
-    const double thick = pl.thickness * pcor;
-    const double xOverX0 = pl.thicknessInX0 * pcor;
-    if (!(0.0 < thick)) return;
+    const S qq = (S)q;
+    const S qOverP = (S)(q / momentum);
+    const S mass = (S)a.mass;
+    const S nr[3] = {(S)pl.r[2], (S)pl.r[5], (S)pl.r[8]};
+    const S d[3] = {(S)dir[0], (S)dir[1], (S)dir[2]};
+    const S pcor = (S)1 / fabs(dot3(nr, d));
+    const S thick = (S)pl.thickness * pcor;
+    const S xOverX0 = (S)pl.thicknessInX0 * pcor;
+    if (!((S)0 < thick)) return;
     // RelativisticQuantities, Interactions.ipp:27-52
-    const double q2OverBeta2 = qq * qq + (mass * qOverP) * (mass * qOverP);
-    const double mOverP = mass * fabs(qOverP / qq);
-    const double pOverM = 1.0 / mOverP;
-    const double beta2 = 1.0 / (1.0 + mOverP * mOverP);
-    const double betaGamma = pOverM;
-    const double gamma = sqrt(1.0 + pOverM * pOverM);
-    const double Ne = pl.molarElectronDensity;
-    const double eps = 0.5 * a.kK * Ne * thick * q2OverBeta2;
-    double Eloss = 0.0, vPhi = 0.0, vTheta = 0.0, vQop = 0.0;
+    const S q2OverBeta2 = qq * qq + (mass * qOverP) * (mass * qOverP);
+    const S mOverP = mass * fabs(qOverP / qq);
+    const S pOverM = (S)1 / mOverP;
+    const S beta2 = (S)1 / ((S)1 + mOverP * mOverP);
+    const S betaGamma = pOverM;
+    const S gamma = sqrt((S)1 + pOverM * pOverM);
+    const S Ne = (S)pl.molarElectronDensity;
+    const S eps = (S)0.5 * (S)a.kK * Ne * thick * q2OverBeta2;
+    S Eloss = (S)0, vPhi = (S)0, vTheta = (S)0, vQop = (S)0;
     if (a.eloss) {
-      const double I = pl.meanExcitation;
-      double dhalf = 0.0;
-      if (!(betaGamma < 10.0)) {
-        const double plasma = a.kPlasma * sqrt(Ne);
-        dhalf = log(betaGamma) + log(plasma / I) - 0.5;
+      const S I = (S)pl.meanExcitation;
+      const S kMe = (S)a.kMe;
+      S dhalf = (S)0;
+      if (!(betaGamma < (S)10)) {
+        const S plasma = (S)a.kPlasma * sqrt(Ne);
+        dhalf = log(betaGamma) + log(plasma / I) - (S)0.5;
       }
-      const double u = 2.0 * a.kMe * betaGamma * betaGamma;
-      const double mfrac = a.kMe / mass;
-      const double nom = 2.0 * a.kMe * betaGamma * betaGamma;
-      const double den = 1.0 + 2.0 * gamma * mfrac + mfrac * mfrac;
-      const double wmax = nom / den;
-      const double running = 0.5 * log(u / I) + 0.5 * log(wmax / I) - beta2 - dhalf;
+      const S u = (S)2 * kMe * betaGamma * betaGamma;
+      const S mfrac = kMe / mass;
+      const S nom = (S)2 * kMe * betaGamma * betaGamma;
+      const S den = (S)1 + (S)2 * gamma * mfrac + mfrac * mfrac;
+      const S wmax = nom / den;
+      const S running = (S)0.5 * log(u / I) + (S)0.5 * log(wmax / I) - beta2 - dhalf;
       Eloss = eps * running;
     }
     if (a.ms) {
-      const double tt = sqrt(xOverX0 * q2OverBeta2);
-      const double theta0 = a.kHighland * fabs(qOverP / qq) * tt * (1.0 + a.kHighlandLog * log(tt));
-      const double dn = sqrt(dir[0] * dir[0] + dir[1] * dir[1] + dir[2] * dir[2]);
-      const double sigmaPhi = theta0 * (dn / sqrt(dir[0] * dir[0] + dir[1] * dir[1]));
+      const S tt = sqrt(xOverX0 * q2OverBeta2);
+      const S theta0 =
+          (S)a.kHighland * fabs(qOverP / qq) * tt * ((S)1 + (S)a.kHighlandLog * log(tt));
+      const S dn = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+      const S sigmaPhi = theta0 * (dn / sqrt(d[0] * d[0] + d[1] * d[1]));
       vPhi = sigmaPhi * sigmaPhi;
       vTheta = theta0 * theta0;
     }
     if (a.eloss) {
-      const double fwhm = 4.0 * eps;
-      const double sigmaE = fwhm / a.kLandauConv;
-      const double pInv = qOverP / qq;
-      const double sq = sqrt(q2OverBeta2) * pInv * pInv * sigmaE;
+      const S fwhm = (S)4 * eps;
+      const S sigmaE = fwhm / (S)a.kLandauConv;
+      const S pInv = qOverP / qq;
+      const S sq = sqrt(q2OverBeta2) * pInv * pInv * sigmaE;
       vQop = sq * sq;
     }
-    const double nextE = sqrt(mass * mass + momentum * momentum) - copysign(Eloss, (double)navDir);
-    p = (mass < nextE) ? sqrt(nextE * nextE - mass * mass) : 0.0;
+    const double nextE =
+        sqrt(a.mass * a.mass + momentum * momentum) - copysign((double)Eloss, (double)navDir);
+    p = (a.mass < nextE) ? sqrt(nextE * nextE - a.mass * a.mass) : 0.0;
     const double mode = (navDir == 1) ? 1.0 : -1.0;
     double v;  // std::max(0, v) of updateVariance
-    v = cov[sidx(2, 2)] + copysign(vPhi, mode);
+    v = cov[sidx(2, 2)] + copysign((double)vPhi, mode);
     cov[sidx(2, 2)] = (0.0 < v) ? v : 0.0;
-    v = cov[sidx(3, 3)] + copysign(vTheta, mode);
+    v = cov[sidx(3, 3)] + copysign((double)vTheta, mode);
     cov[sidx(3, 3)] = (0.0 < v) ? v : 0.0;
-    v = cov[sidx(4, 4)] + copysign(vQop, mode);
+    v = cov[sidx(4, 4)] + copysign((double)vQop, mode);
     cov[sidx(4, 4)] = (0.0 < v) ? v : 0.0;
   }
 
   // -------------------------------------------------------------- filter ----
   // Actor::filter (KalmanFitter.hpp:284-350) with GainMatrixUpdater
-  // (GainMatrixUpdater.hpp:34-106): (I - K H) P update.
+  // (GainMatrixUpdater.hpp:34-106).  FP64 mode: the reference's (I - K H) P
+  // update.  Mixed mode: the Joseph form (I - K H) P (I - K H)^T + K V K^T,
+  // which stays positive definite under the float-rounded predicted
+  // covariance; each (i, j) pair is formed once, so the result is symmetric
+  // by construction.  The 2x2 inversion and the update are double in both.
   KFD_INLINE bool filter(int sp) {
     const DevPlane &pl = plane(sp);
     int64_t hit = -1;
@@ -734,15 +755,22 @@ struct TrackFit {
     double xp[6];
     S jb[6][6];
     boundStatePlane(pl, xp, jb);
-    // predicted state + jacobian to the store
-    storeBound(k, a.off_pred, xp, cov);
+    // predicted parameters and the transport Jacobian to the store
+#pragma unroll
+    for (int m = 0; m < 6; ++m) *recD(k, KF_SD_XP + m) = xp[m];
 #pragma unroll
     for (int r = 0; r < 6; ++r)
 #pragma unroll
-      for (int c = 0; c < 6; ++c) *rec(k, a.off_jac + 6 * r + c) = (double)jb[r][c];
-    if (a.full_record) {
-      *rec(k, KF_TS_PATH) = pathAcc;
-      *rec(k, KF_TS_GEO_ID) = (double)pl.geo_id;
+      for (int c = 0; c < 5; ++c)
+        if (r != 4) *recS(k, KF_SS_J + jidx(r, c)) = jb[r][c];
+    if (tsf) {
+      storeBoundF(k, KF_TS_PREDICTED, xp, cov);
+#pragma unroll
+      for (int r = 0; r < 6; ++r)
+#pragma unroll
+        for (int c = 0; c < 6; ++c) *recF(k, KF_TS_JACOBIAN + 6 * r + c) = (double)jb[r][c];
+      *recF(k, KF_TS_PATH) = pathAcc;
+      *recF(k, KF_TS_GEO_ID) = (double)pl.geo_id;
     }
     // gain-matrix update
     const double m0 = a.hit_meas[hit], m1 = a.hit_meas[a.n_hits + hit];
@@ -763,23 +791,52 @@ struct TrackFit {
 #pragma unroll
     for (int r = 0; r < 6; ++r) xf[r] = xp[r] + (K0[r] * r0 + K1[r] * r1);
     double cf[21];
+    if constexpr (kMixed) {
+      // B = (I - K H) P, needed at (r, c) with r <= c or c < 2
+      double B[6][6];
 #pragma unroll
-    for (int r = 0; r < 6; ++r)
+      for (int r = 0; r < 6; ++r)
 #pragma unroll
-      for (int c = r; c < 6; ++c) {
-        const double ca = ((r == 0) ? 1.0 : 0.0) - K0[r];
-        const double cb = ((r == 1) ? 1.0 : 0.0) - K1[r];
-        double s = ca * cov[sidx(0, c)] + cb * cov[sidx(1, c)];
-        if (r >= 2) s = s + cov[sidx(r, c)];
-        cf[sidx(r, c)] = s;
-      }
+        for (int c = 0; c < 6; ++c)
+          if (r <= c || c < 2) B[r][c] = cov[sidx(r, c)] - K0[r] * cov[sidx(0, c)] - K1[r] * cov[sidx(1, c)];
+#pragma unroll
+      for (int r = 0; r < 6; ++r)
+#pragma unroll
+        for (int c = r; c < 6; ++c) {
+          const double kv0 = V00 * K0[c] + V01 * K1[c];
+          const double kv1 = V01 * K0[c] + V11 * K1[c];
+          cf[sidx(r, c)] = (B[r][c] - B[r][0] * K0[c] - B[r][1] * K1[c]) + (K0[r] * kv0 + K1[r] * kv1);
+        }
+    } else {
+#pragma unroll
+      for (int r = 0; r < 6; ++r)
+#pragma unroll
+        for (int c = r; c < 6; ++c) {
+          const double ca = ((r == 0) ? 1.0 : 0.0) - K0[r];
+          const double cb = ((r == 1) ? 1.0 : 0.0) - K1[r];
+          double s = ca * cov[sidx(0, c)] + cb * cov[sidx(1, c)];
+          if (r >= 2) s = s + cov[sidx(r, c)];
+          cf[sidx(r, c)] = s;
+        }
+    }
     const double chi2 = r0 * (i00 * r0 + i01 * r1) + r1 * (i10 * r0 + i11 * r1);
     chi2sum = chi2sum + chi2;
     correctParams(xf);
-    storeBound(k, a.off_filt, xf, cf);
-    if (a.full_record) *rec(k, KF_TS_CHI2) = chi2;
+#pragma unroll
+    for (int m = 0; m < 6; ++m) *recD(k, KF_SD_XF + m) = xf[m];
+#pragma unroll
+    for (int m = 0; m < 21; ++m) *recS(k, KF_SS_CF + m) = (S)cf[m];
+    if (tsf) {
+      storeBoundF(k, KF_TS_FILTERED, xf, cf);
+      *recF(k, KF_TS_CHI2) = chi2;
+    }
     updateFromBound(pl, xf, cf);
     material(pl);
+    // the covariance the next transport starts from differs from the stored
+    // filtered one only on these three diagonal entries
+    *recS(k, KF_SS_MD + 0) = (S)cov[sidx(2, 2)];
+    *recS(k, KF_SS_MD + 1) = (S)cov[sidx(3, 3)];
+    *recS(k, KF_SS_MD + 2) = (S)cov[sidx(4, 4)];
     ++measured;
     return true;
   }
@@ -846,36 +903,76 @@ struct TrackFit {
     }
   }
 
+  // Filtered covariance of state k, in double.
+  KFD_INLINE void loadFilteredCov(int k, double c[21]) const {
+#pragma unroll
+    for (int m = 0; m < 21; ++m) c[m] = (double)*recS(k, KF_SS_CF + m);
+  }
+
+  // Transport Jacobian of state k (rows 0, 1, 2, 3, 5; row 4 = e4, col 5 = e5).
+  KFD_INLINE void loadJacobian(int k, S J[6][6]) const {
+#pragma unroll
+    for (int r = 0; r < 6; ++r)
+#pragma unroll
+      for (int c = 0; c < 6; ++c) {
+        if (r == 4 || c == 5)
+          J[r][c] = (r == c) ? (S)1 : (S)0;
+        else
+          J[r][c] = *recS(k, KF_SS_J + jidx(r, c));
+      }
+  }
+
+  // Predicted covariance of state k + 1: the forward pass's transport of the
+  // post-material covariance of state k, recomputed from the same operands.
+  KFD_INLINE void predictedCov(int k, const S J[6][6], double cp[21]) const {
+    S cm[21];
+#pragma unroll
+    for (int m = 0; m < 21; ++m) cm[m] = *recS(k, KF_SS_CF + m);
+    cm[sidx(2, 2)] = *recS(k, KF_SS_MD + 0);
+    cm[sidx(3, 3)] = *recS(k, KF_SS_MD + 1);
+    cm[sidx(4, 4)] = *recS(k, KF_SS_MD + 2);
+    sandwich(J, cm, cp);
+  }
+
   // GainMatrixSmoother (GainMatrixSmoother.hpp:21-94):
-  // G = (C_f J^T) C_p^-1, x_s = x_f + G (x_s' - x_p'), C_s = C_f - G (C_p - C_s') G^T
+  // G = (C_f J^T) C_p^-1, x_s = x_f + G (x_s' - x_p'), C_s = C_f - G (C_p - C_s') G^T,
+  // in double in both modes.
   KFD_INLINE bool smooth(double xs[6], double cs[21]) {
     const int n = measured;
     double xsn[6], csn[21];
-    loadBound(n - 1, a.off_filt, xsn, csn);  // last state: smoothed = filtered
-    if (a.full_record) storeBound(n - 1, KF_TS_SMOOTHED, xsn, csn);
+    // last state: smoothed = filtered
+#pragma unroll
+    for (int m = 0; m < 6; ++m) xsn[m] = *recD(n - 1, KF_SD_XF + m);
+    loadFilteredCov(n - 1, csn);
+    if (tsf) storeBoundF(n - 1, KF_TS_SMOOTHED, xsn, csn);
     for (int k = n - 2; k >= 0; --k) {
       double G[6][6];
+      double Cp[21];
       {
+        S J[6][6];
+        loadJacobian(k + 1, J);
+        predictedCov(k, J, Cp);
+        KFD_PHASE_FENCE();
         double lu[6][6];
         int perm[6];
 #pragma unroll
         for (int r = 0; r < 6; ++r)
 #pragma unroll
-          for (int c = 0; c < 6; ++c) lu[r][c] = *rec(k + 1, a.off_pred + 6 + sidx(r, c));
+          for (int c = 0; c < 6; ++c) lu[r][c] = Cp[sidx(r, c)];
         lu6(lu, perm);
-        KFD_PHASE_FENCE();
         double A1[6][6];  // C_f J^T
+        {
+          double cf[21];
+          loadFilteredCov(k, cf);
 #pragma unroll
-        for (int c = 0; c < 6; ++c) {
-          double jr[6];
+          for (int c = 0; c < 6; ++c) {
 #pragma unroll
-          for (int m = 0; m < 6; ++m) jr[m] = *rec(k + 1, a.off_jac + 6 * c + m);
+            for (int r = 0; r < 6; ++r) {
+              double s = cf[sidx(r, 0)] * (double)J[c][0];
 #pragma unroll
-          for (int r = 0; r < 6; ++r) {
-            double s = *rec(k, a.off_filt + 6 + sidx(r, 0)) * jr[0];
-#pragma unroll
-            for (int m = 1; m < 6; ++m) s = s + *rec(k, a.off_filt + 6 + sidx(r, m)) * jr[m];
-            A1[r][c] = s;
+              for (int m = 1; m < 6; ++m) s = s + cf[sidx(r, m)] * (double)J[c][m];
+              A1[r][c] = s;
+            }
           }
         }
         KFD_PHASE_FENCE();
@@ -904,18 +1001,20 @@ struct TrackFit {
       KFD_PHASE_FENCE();
       double dx[6];
 #pragma unroll
-      for (int r = 0; r < 6; ++r) dx[r] = xsn[r] - *rec(k + 1, a.off_pred + r);
+      for (int r = 0; r < 6; ++r) dx[r] = xsn[r] - *recD(k + 1, KF_SD_XP + r);
 #pragma unroll
       for (int r = 0; r < 6; ++r) {
         double s = G[r][0] * dx[0];
 #pragma unroll
         for (int m = 1; m < 6; ++m) s = s + G[r][m] * dx[m];
-        xsn[r] = *rec(k, a.off_filt + r) + s;
+        xsn[r] = *recD(k, KF_SD_XF + r) + s;
       }
       KFD_PHASE_FENCE();
       double Dm[21];
 #pragma unroll
-      for (int m = 0; m < 21; ++m) Dm[m] = *rec(k + 1, a.off_pred + 6 + m) - csn[m];
+      for (int m = 0; m < 21; ++m) Dm[m] = Cp[m] - csn[m];
+      double cf[21];
+      loadFilteredCov(k, cf);
 #pragma unroll
       for (int r = 0; r < 6; ++r) {
         double B1[6];  // row r of G (C_p - C_s')
@@ -931,11 +1030,11 @@ struct TrackFit {
           double s = B1[0] * G[c][0];
 #pragma unroll
           for (int m = 1; m < 6; ++m) s = s + B1[m] * G[c][m];
-          csn[sidx(r, c)] = *rec(k, a.off_filt + 6 + sidx(r, c)) - s;
+          csn[sidx(r, c)] = cf[sidx(r, c)] - s;
         }
       }
       correctParams(xsn);
-      if (a.full_record) storeBound(k, KF_TS_SMOOTHED, xsn, csn);
+      if (tsf) storeBoundF(k, KF_TS_SMOOTHED, xsn, csn);
     }
 #pragma unroll
     for (int m = 0; m < 6; ++m) xs[m] = xsn[m];
@@ -1208,37 +1307,47 @@ struct TrackFit {
   }
 };
 
-// Base of the state store of the 64-track chunk this wave works on.  Blocks
-// are multiples of 64 work-items, so a wave covers exactly one chunk.
-KFD_INLINE double *chunkBase(const FitArgs &a, int64_t i) {
-  const uint32_t chunk = __builtin_amdgcn_readfirstlane((uint32_t)(i >> 6));
-  return a.ts + (size_t)chunk * (size_t)a.max_states * (size_t)a.ts_fields * 64u;
-}
+// Bases of the state stores of the 64-track chunk this wave works on.
+// Blocks are multiples of 64 work-items, so a wave covers exactly one chunk.
+template <typename S>
+struct ChunkBases {
+  double *d;
+  S *s;
+  double *f;
+  KFD_INLINE ChunkBases(const FitArgs &a, int64_t i) {
+    const size_t chunk = __builtin_amdgcn_readfirstlane((uint32_t)(i >> 6));
+    const size_t per = (size_t)a.max_states * 64u;
+    d = a.tsd + chunk * per * KF_SD_N;
+    s = (S *)a.tss + chunk * per * KF_SS_N;
+    f = a.tsf ? a.tsf + chunk * per * KF_TS_STRIDE : nullptr;
+  }
+};
 
 template <typename S>
 __global__ void __launch_bounds__(KF_BLOCK) kf_forward_kernel(FitArgs args) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  double *base = chunkBase(args, i);
+  const ChunkBases<S> cb(args, i);
   if (i >= args.n) return;
-  TrackFit<S, true> tf(args, i, base, (uint32_t)(i & 63));
+  TrackFit<S, true> tf(args, i, cb.d, cb.s, cb.f, (uint32_t)(i & 63));
   tf.runForward();
 }
 
 template <typename S>
 __global__ void __launch_bounds__(KF_BLOCK) kf_backward_kernel(FitArgs args) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  double *base = chunkBase(args, i);
+  const ChunkBases<S> cb(args, i);
   if (i >= args.n) return;
-  TrackFit<S, false> tf(args, i, base, (uint32_t)(i & 63));
+  TrackFit<S, false> tf(args, i, cb.d, cb.s, cb.f, (uint32_t)(i & 63));
   tf.runBackward();
 }
 
-// Copies the state store into kf_results.track_states ([state][120][n]).
+// Copies the public track-state records into kf_results.track_states
+// ([state][KF_TS_STRIDE][n]).
 __global__ void __launch_bounds__(256) kf_export_states(FitArgs args, double *out) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= args.n) return;
   const int nm = args.out_nmeas[i];
-  const double *base = args.ts + (size_t)(i >> 6) * (size_t)args.max_states * KF_TS_STRIDE * 64u;
+  const double *base = args.tsf + (size_t)(i >> 6) * (size_t)args.max_states * KF_TS_STRIDE * 64u;
   const uint32_t lane = (uint32_t)(i & 63);
   for (int k = 0; k < nm && k < args.max_states; ++k)
     for (int f = 0; f < KF_TS_STRIDE; ++f)
@@ -1249,20 +1358,22 @@ __global__ void __launch_bounds__(256) kf_export_states(FitArgs args, double *ou
 }  // namespace kfd
 
 extern "C" hipError_t kfd_launch_fit(const kfd::FitArgs *args, int precision, hipStream_t stream,
-                                     double *track_states_out) {
+                                     double *track_states_out, hipEvent_t between) {
   const int64_t blocks = (args->n + KF_BLOCK - 1) / KF_BLOCK;
   if (blocks <= 0) return hipSuccess;
-  if (precision == KF_PRECISION_FP64) {
-    hipLaunchKernelGGL(kfd::kf_forward_kernel<double>, dim3((unsigned)blocks), dim3(KF_BLOCK), 0,
-                       stream, *args);
-    hipLaunchKernelGGL(kfd::kf_backward_kernel<double>, dim3((unsigned)blocks), dim3(KF_BLOCK), 0,
-                       stream, *args);
-  } else {
-    hipLaunchKernelGGL(kfd::kf_forward_kernel<float>, dim3((unsigned)blocks), dim3(KF_BLOCK), 0,
-                       stream, *args);
-    hipLaunchKernelGGL(kfd::kf_backward_kernel<float>, dim3((unsigned)blocks), dim3(KF_BLOCK), 0,
-                       stream, *args);
+  const dim3 grid((unsigned)blocks), block(KF_BLOCK);
+  if (precision == KF_PRECISION_FP64)
+    hipLaunchKernelGGL(kfd::kf_forward_kernel<double>, grid, block, 0, stream, *args);
+  else
+    hipLaunchKernelGGL(kfd::kf_forward_kernel<float>, grid, block, 0, stream, *args);
+  if (between) {
+    const hipError_t e = hipEventRecord(between, stream);
+    if (e != hipSuccess) return e;
   }
+  if (precision == KF_PRECISION_FP64)
+    hipLaunchKernelGGL(kfd::kf_backward_kernel<double>, grid, block, 0, stream, *args);
+  else
+    hipLaunchKernelGGL(kfd::kf_backward_kernel<float>, grid, block, 0, stream, *args);
   if (track_states_out) {
     const int64_t eb = (args->n + 255) / 256;
     hipLaunchKernelGGL(kfd::kf_export_states, dim3((unsigned)eb), dim3(256), 0, stream, *args,

@@ -9,6 +9,15 @@
 
 #define KF_BLOCK 128  // multiple of 64: a wave covers one 64-track store chunk
 
+// per-state store fields (see FitArgs)
+#define KF_SD_XP 0   // predicted parameters (6)
+#define KF_SD_XF 6   // filtered parameters (6)
+#define KF_SD_N 12
+#define KF_SS_CF 0   // filtered covariance (21, packed upper triangle)
+#define KF_SS_MD 21  // covariance (2,2), (3,3), (4,4) after the material update
+#define KF_SS_J 24   // Jacobian rows 0, 1, 2, 3, 5 x columns 0..4 (25)
+#define KF_SS_N 49
+
 namespace kfd {
 
 // Kernel argument block (passed by value).  All pointers are device memory.
@@ -41,12 +50,22 @@ struct FitArgs {
   int32_t *out_ndf;
   int32_t *out_nmeas;
   uint32_t *out_status;
-  // per-state store, chunked by 64 tracks: element f of state k of track i at
-  // ts[((i / 64) * max_states * ts_fields + k * ts_fields + f) * 64 + i % 64]
-  double *ts;
-  int32_t ts_fields;
-  int32_t off_pred, off_filt, off_jac;
-  int32_t full_record;  // 1: fields use the KF_TS_* layout (120), exported after the fit
+  // Per-state store of the smoother, chunked by 64 tracks: element f of
+  // state k of track i sits at
+  //   base[((i / 64) * max_states * F + k * F + f) * 64 + i % 64].
+  //  tsd  double, F = KF_SD_N: predicted and filtered parameters;
+  //  tss  the transport scalar S (double in FP64 mode, float in mixed mode),
+  //       F = KF_SS_N: filtered covariance, the three variances after the
+  //       material update, and the 25 free entries of the bound-to-bound
+  //       transport Jacobian (row 4 is e4, column 5 is e5 on every surface);
+  //  tsf  double, F = KF_TS_STRIDE: the public track-state record, written
+  //       only when the caller asks for track states (else NULL).
+  // The predicted covariance is not stored: the smoother recomputes it from
+  // the stored Jacobian and post-material covariance with the same operation
+  // sequence, so it is bit-identical to the forward pass's.
+  double *tsd;
+  void *tss;
+  double *tsf;
   // forward -> backward kernel hand-over, [field][n]
   int32_t *carry_i;  // 6 int fields
   double *carry_d;   // 30 double fields
@@ -54,5 +73,8 @@ struct FitArgs {
 
 }  // namespace kfd
 
+// Launches the forward kernel, records `between` (if not NULL), launches the
+// smoother/back-propagation kernel, then the track-state export if
+// track_states_out is not NULL.
 extern "C" hipError_t kfd_launch_fit(const kfd::FitArgs *args, int precision, hipStream_t stream,
-                                     double *track_states_out);
+                                     double *track_states_out, hipEvent_t between);

@@ -52,6 +52,8 @@ def _lib():
         lib.kf_fit.restype = C.c_int
         lib.kf_last_kernel_ms.argtypes = [P]
         lib.kf_last_kernel_ms.restype = C.c_double
+        lib.kf_last_phase_ms.argtypes = [P, P, P]
+        lib.kf_last_phase_ms.restype = C.c_int
         lib._kf_typed = True
     return lib
 
@@ -275,7 +277,14 @@ class KalmanFitter:
                                      C.c_void_p(s.cuda_stream)))
 
     def last_kernel_ms(self):
+        """Duration of the last fit's kernels (both launches), ms."""
         return float(self._lib.kf_last_kernel_ms(self._ctx))
+
+    def last_phase_ms(self):
+        """(forward filter, smoother + back-propagation) durations of the last fit, ms."""
+        f, b = C.c_double(), C.c_double()
+        self._check(self._lib.kf_last_phase_ms(self._ctx, C.byref(f), C.byref(b)))
+        return f.value, b.value
 
     # ------------------------------------------------------- per-track API --
     def fit(self, sourcelinks, start_parameters, kf_options, perigee,

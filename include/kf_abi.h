@@ -190,6 +190,14 @@ int kf_fit(kf_ctx *ctx, const kf_tracks *tracks, const kf_options *opt,
  * on the launch stream); < 0 if unknown. */
 double kf_last_kernel_ms(const kf_ctx *ctx);
 
+/* The same split at the hand-over between the two kernels of a fit: the
+ * forward filter (up to the reference's finalize()) and the smoother plus
+ * back-propagation to the perigee.  The backward time includes the
+ * track-state export when track states were requested.  No reference
+ * counterpart (the reference times the whole batch from the host,
+ * Run/GPU/KalmanFitterGPUTest.cu:382-513). */
+int kf_last_phase_ms(const kf_ctx *ctx, double *forward_ms, double *backward_ms);
+
 #ifdef __cplusplus
 }
 #endif

@@ -40,7 +40,7 @@ def _lib(counting=False):
     P = C.c_void_p
     lib.kfo_fit_batch.argtypes = [P, C.c_int32, C.POINTER(kfo_field), C.POINTER(abi.kf_tracks),
                                   C.POINTER(abi.kf_options), C.POINTER(abi.kf_results),
-                                  C.c_int32, C.c_int32, P, P, P]
+                                  C.c_int32, C.c_int32, P, P, P, P]
     lib.kfo_fit_batch.restype = C.c_int
     lib.kfo_time_fit_batch.argtypes = [P, C.c_int32, C.POINTER(kfo_field), C.POINTER(abi.kf_tracks),
                                        C.POINTER(abi.kf_options), C.POINTER(abi.kf_results), C.c_int32]
@@ -118,7 +118,11 @@ def host_tracks(batch):
 
 def fit(batch, options=None, track_states=False, n_threads=0, counting=False, planes=None,
         field=None):
-    """Fits ``batch`` with the oracle.  Returns (HostResults, steps, flops, trans)."""
+    """Fits ``batch`` with the oracle.  Returns (HostResults, steps, flops, trans);
+    with ``counting`` the results also carry ``flops32``, the part of each
+    track's flops in the float class of the mixed mode, and ``flops_bwd``,
+    the part from the smoother on (the HIP fitter's second kernel), and
+    ``flops32_bwd``, the float-class part of the latter."""
     lib = _lib(counting)
     opts = options if options is not None else abi.default_options()
     n = batch.n_tracks
@@ -127,13 +131,15 @@ def fit(batch, options=None, track_states=False, n_threads=0, counting=False, pl
     steps = np.zeros(n, dtype=np.int32)
     flops = np.zeros(n, dtype=np.uint64)
     trans = np.zeros(n, dtype=np.uint64)
+    classes = np.zeros((3, n), dtype=np.uint64)
     pl = planes if planes is not None else batch.planes
     fl = make_field(field if field is not None else batch.field)
     tr = host_tracks(batch)
     rc = lib.kfo_fit_batch(pl, len(pl), C.byref(fl), C.byref(tr), C.byref(opts), C.byref(res.c()),
-                           max_states, n_threads, _p(steps), _p(flops), _p(trans))
+                           max_states, n_threads, _p(steps), _p(flops), _p(trans), _p(classes))
     if rc != 0:
         raise RuntimeError(f"kfo_fit_batch failed ({rc})")
+    res.flops32, res.flops_bwd, res.flops32_bwd = classes[0], classes[1], classes[2]
     return res, steps, flops, trans
 
 

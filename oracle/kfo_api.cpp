@@ -19,7 +19,12 @@ using namespace kfo;
 #ifdef KFO_COUNTING
 namespace kfo {
 thread_local uint64_t g_flops = 0;
+thread_local uint64_t g_flops32 = 0;
 thread_local uint64_t g_trans = 0;
+thread_local int g_in32 = 0;
+thread_local uint64_t g_flops_bwd = 0;
+thread_local uint64_t g_flops32_bwd = 0;
+thread_local int g_bwd = 0;
 }  // namespace kfo
 #endif
 
@@ -133,11 +138,14 @@ extern "C" {
 // Fits n tracks (host memory).  track_states, when not NULL, receives
 // [max_states][KF_TS_STRIDE][n] like kf_results.track_states; steps (may be
 // NULL) the number of propagation steps of each track; flops / trans (may be
-// NULL; counting build only) the counted flops and transcendentals per track.
+// NULL; counting build only) the counted flops and transcendentals per track,
+// flop_classes ([3][n]) the part of the flops in the float class of the
+// mixed mode, the part from the smoother on, and the float-class part of
+// the latter.
 int kfo_fit_batch(const kf_plane *planes, int32_t n_planes, const kfo_field *field,
                   const kf_tracks *tr, const kf_options *opt, const kf_results *res,
                   int32_t max_states, int32_t n_threads, int32_t *steps,
-                  uint64_t *flops, uint64_t *trans) {
+                  uint64_t *flops, uint64_t *trans, uint64_t *flop_classes) {
   if (!planes || !field || !tr || !opt || !res) return KF_E_INVALID;
   const int64_t n = tr->n_tracks;
   std::vector<Surface> surf(n_planes);
@@ -150,7 +158,9 @@ int kfo_fit_batch(const kf_plane *planes, int32_t n_planes, const kfo_field *fie
 #pragma omp parallel for num_threads(n_threads) schedule(dynamic, 64)
   for (int64_t it = 0; it < n; ++it) {
 #ifdef KFO_COUNTING
-    const uint64_t f0 = g_flops, t0 = g_trans;
+    const uint64_t f0 = g_flops, t0 = g_trans, g0 = g_flops32, b0 = g_flops_bwd,
+                   c0 = g_flops32_bwd;
+    g_bwd = 0;
 #endif
     const int first = tr->seq_first[it], count = tr->seq_count[it];
     std::vector<const Surface *> seq(count > 0 ? count : 0);
@@ -229,9 +239,15 @@ int kfo_fit_batch(const kf_plane *planes, int32_t n_planes, const kfo_field *fie
 #ifdef KFO_COUNTING
     if (flops) flops[it] = g_flops - f0;
     if (trans) trans[it] = g_trans - t0;
+    if (flop_classes) {
+      flop_classes[it] = g_flops32 - g0;
+      flop_classes[n + it] = g_flops_bwd - b0;
+      flop_classes[2 * n + it] = g_flops32_bwd - c0;
+    }
 #else
     (void)flops;
     (void)trans;
+    (void)flop_classes;
 #endif
   }
   return KF_OK;
@@ -245,7 +261,7 @@ double kfo_time_fit_batch(const kf_plane *planes, int32_t n_planes,
                           int32_t n_threads) {
   auto t0 = std::chrono::steady_clock::now();
   int rc = kfo_fit_batch(planes, n_planes, field, tr, opt, res, 0, n_threads,
-                         nullptr, nullptr, nullptr);
+                         nullptr, nullptr, nullptr, nullptr);
   auto t1 = std::chrono::steady_clock::now();
   if (rc != KF_OK) return -1.0;
   return std::chrono::duration<double>(t1 - t0).count();

@@ -693,6 +693,7 @@ class Stepper {
     if (s.covTransport) s.derivative[3] = dtds;
 
     if (s.covTransport) {
+      F32Region f32;
       M88 D = transportMatrix(st, B_first, B_middle, B_last, k1, k2, k3, h);
       s.jacTransport = D * s.jacTransport;
     }
@@ -766,16 +767,22 @@ class Stepper {
   // CovarianceEngine.hpp:106-125, 169-201, 269-290
   void covarianceTransport(StepperState &s, const V8 &freePars,
                            const Surface &surf) const {
-    s.jacToGlobal = s.jacTransport * s.jacToGlobal;
+    {
+      F32Region f32;
+      s.jacToGlobal = s.jacTransport * s.jacToGlobal;
+    }
     M68 jacToLocal = M68::zero();
     V3 pos = v3(freePars[0], freePars[1], freePars[2]);
     V3 dir = v3(freePars[4], freePars[5], freePars[6]);
     M33 rft = initJacobianToLocal(surf, jacToLocal, pos, dir);
     R6 sVec = derivativeFactors(surf, pos, dir, rft, s.jacToGlobal);
     Mat<8, 1> dv = s.derivative;
-    s.jacToGlobal = s.jacToGlobal - dv * sVec;
-    s.jacobian = jacToLocal * s.jacToGlobal;
-    s.cov = s.jacobian * s.cov * transpose(s.jacobian);
+    {
+      F32Region f32;
+      s.jacToGlobal = s.jacToGlobal - dv * sVec;
+      s.jacobian = jacToLocal * s.jacToGlobal;
+      s.cov = s.jacobian * s.cov * transpose(s.jacobian);
+    }
     // reinitializeJacobians
     s.jacTransport = M88::identity();
     s.derivative = V8::zero();
@@ -1084,21 +1091,24 @@ struct KalmanFitter {
       else
         slab.scaleThickness(Real(factor));
     }
-    Real pc = pathCorrection(*surface, dir);
-    slab.scaleThickness(pc);
-    if (!slab) return;
     Real Eloss = 0.0, variancePhi = 0.0, varianceTheta = 0.0, varianceQoverP = 0.0;
-    if (fo.energyLoss) Eloss = energyLossBethe(slab, mass, qOverP, q);
-    if (s.covTransport) {
-      if (fo.multipleScattering) {
-        const Real theta0 = multipleScatteringTheta0(slab, mass, qOverP, q);
-        const Real sigmaPhi = theta0 * (norm(dir) / ksqrt(dir[0] * dir[0] + dir[1] * dir[1]));
-        variancePhi = sigmaPhi * sigmaPhi;
-        varianceTheta = theta0 * theta0;
-      }
-      if (fo.energyLoss) {
-        const Real sq = landauSigmaQOverP(slab, mass, qOverP, q);
-        varianceQoverP = sq * sq;
+    {
+      F32Region f32;  // the material formulas, not the state update below
+      Real pc = pathCorrection(*surface, dir);
+      slab.scaleThickness(pc);
+      if (!slab) return;
+      if (fo.energyLoss) Eloss = energyLossBethe(slab, mass, qOverP, q);
+      if (s.covTransport) {
+        if (fo.multipleScattering) {
+          const Real theta0 = multipleScatteringTheta0(slab, mass, qOverP, q);
+          const Real sigmaPhi = theta0 * (norm(dir) / ksqrt(dir[0] * dir[0] + dir[1] * dir[1]));
+          variancePhi = sigmaPhi * sigmaPhi;
+          varianceTheta = theta0 * theta0;
+        }
+        if (fo.energyLoss) {
+          const Real sq = landauSigmaQOverP(slab, mass, qOverP, q);
+          varianceQoverP = sq * sq;
+        }
       }
     }
     // updateState
@@ -1137,6 +1147,7 @@ struct KalmanFitter {
 
   // Actor::finalize, KalmanFitter.hpp:360-393
   bool finalize(PropState &st, FitResult &r) const {
+    kfo_mark_backward();
     r.smoothed = true;
     bool nanFound = false;
     const BoundParams *sp = smooth(r.states, r.measurementStates, nanFound);

@@ -11,6 +11,12 @@
 // products (D, jacToLocal, jacToGlobal, H) so the count is the algorithmic
 // flop count of the fit.  Transcendentals (sin cos atan2 log pow hypot exp)
 // are counted in a separate counter; sqrt counts as one flop.
+//
+// Flops inside an F32Region are also counted in g_flops32: the operations
+// the HIP fitter's mixed-precision mode runs in float (transport Jacobian,
+// bound-state covariance transport, material formulas).  Flops after
+// kfo_mark_backward() (called where the smoother starts) are also counted in
+// g_flops_bwd: the share of the HIP fitter's second kernel.
 #pragma once
 
 #include <cfloat>
@@ -22,7 +28,25 @@ namespace kfo {
 #ifdef KFO_COUNTING
 
 extern thread_local uint64_t g_flops;
+extern thread_local uint64_t g_flops32;
 extern thread_local uint64_t g_trans;
+extern thread_local int g_in32;
+extern thread_local uint64_t g_flops_bwd;
+extern thread_local uint64_t g_flops32_bwd;
+extern thread_local int g_bwd;
+
+inline void kfo_count() {
+  ++g_flops;
+  if (g_in32) ++g_flops32;
+  if (g_bwd) ++g_flops_bwd;
+  if (g_in32 && g_bwd) ++g_flops32_bwd;
+}
+inline void kfo_mark_backward() { g_bwd = 1; }
+
+struct F32Region {
+  F32Region() { ++g_in32; }  // NOLINT
+  ~F32Region() { --g_in32; }
+};
 
 struct Counted {
   double v;
@@ -34,19 +58,19 @@ struct Counted {
 inline bool kfo_unit(double x) { return x == 1.0 || x == -1.0 || x == 0.0; }
 
 inline Counted operator+(Counted a, Counted b) {
-  if (a.v != 0.0 && b.v != 0.0) ++g_flops;
+  if (a.v != 0.0 && b.v != 0.0) kfo_count();
   return Counted(a.v + b.v);
 }
 inline Counted operator-(Counted a, Counted b) {
-  if (a.v != 0.0 && b.v != 0.0) ++g_flops;
+  if (a.v != 0.0 && b.v != 0.0) kfo_count();
   return Counted(a.v - b.v);
 }
 inline Counted operator*(Counted a, Counted b) {
-  if (!kfo_unit(a.v) && !kfo_unit(b.v)) ++g_flops;
+  if (!kfo_unit(a.v) && !kfo_unit(b.v)) kfo_count();
   return Counted(a.v * b.v);
 }
 inline Counted operator/(Counted a, Counted b) {
-  if (a.v != 0.0 && b.v != 1.0 && b.v != -1.0) ++g_flops;
+  if (a.v != 0.0 && b.v != 1.0 && b.v != -1.0) kfo_count();
   return Counted(a.v / b.v);
 }
 inline Counted operator-(Counted a) { return Counted(-a.v); }
@@ -62,7 +86,7 @@ inline bool operator==(Counted a, Counted b) { return a.v == b.v; }
 inline bool operator!=(Counted a, Counted b) { return a.v != b.v; }
 
 inline double kval(Counted x) { return x.v; }
-inline Counted ksqrt(Counted x) { ++g_flops; return Counted(std::sqrt(x.v)); }
+inline Counted ksqrt(Counted x) { kfo_count(); return Counted(std::sqrt(x.v)); }
 inline Counted kabs(Counted x) { return Counted(std::fabs(x.v)); }
 inline Counted kfloor(Counted x) { return Counted(std::floor(x.v)); }
 inline Counted kcopysign(Counted a, Counted b) { return Counted(std::copysign(a.v, b.v)); }
@@ -94,6 +118,11 @@ inline double khypot(double x, double y) { return std::hypot(x, y); }
 inline bool kisnan(double x) { return std::isnan(x); }
 
 using Real = double;
+
+struct F32Region {
+  F32Region() {}  // user-provided: a scope marker, not an unused variable
+};
+inline void kfo_mark_backward() {}
 
 #endif
 

@@ -43,9 +43,10 @@ def compare_fit(gres, ores, mask=None):
             float(c2.max()) if c2.size else 0.0)
 
 
-def compare_states(gts, ots, n_meas, smoothed_mask):
+def compare_states(gts, ots, n_meas, smoothed_mask, chi2_floor=1e-12):
     """Max normalised deviation of predicted / filtered / smoothed states,
-    jacobians, path and chi2 over the measured states of every track."""
+    jacobians, path and chi2 over the measured states of every track.  A
+    state's chi2 deviation is taken against max(chi2, chi2_floor)."""
     worst = {}
     n_states = gts.shape[0]
     for k in range(n_states):
@@ -68,10 +69,10 @@ def compare_states(gts, ots, n_meas, smoothed_mask):
         rowscale = np.maximum(np.abs(oj3).max(axis=1, keepdims=True), 1e-300)
         jd = np.abs(gj.reshape(6, 6, -1) - oj3) / rowscale
         worst["jacobian"] = max(worst.get("jacobian", 0.0), float(jd.max()))
-        for name, f in (("path", abi.KF_TS_PATH), ("chi2", abi.KF_TS_CHI2)):
+        for name, f, floor in (("path", abi.KF_TS_PATH, 1e-12), ("chi2", abi.KF_TS_CHI2, chi2_floor)):
             g, o = gts[k, f][m], ots[k, f][m]
             worst[name] = max(worst.get(name, 0.0),
-                              float((np.abs(g - o) / np.maximum(np.abs(o), 1e-12)).max()))
+                              float((np.abs(g - o) / np.maximum(np.abs(o), floor)).max()))
         worst["geo_id"] = max(worst.get("geo_id", 0.0),
                               float(np.abs(gts[k, abi.KF_TS_GEO_ID][m] - ots[k, abi.KF_TS_GEO_ID][m]).max()))
     return worst

@@ -12,9 +12,16 @@ import oracle
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
-def test_flops_per_track_constant(built):
+def test_flops_per_track_constants(built):
     src = open(os.path.join(ROOT, "bench.py")).read()
-    const = float(re.search(r"^FLOPS_PER_TRACK = ([0-9.e+]+)", src, re.M).group(1))
+
+    def const(name):
+        return float(re.search(rf"^{name} = ([0-9.e+]+)", src, re.M).group(1))
+
     b = G.generate(2, n_tracks=20000)
-    _, _, flops, _ = oracle.fit(b, abi.default_options(**configs.WORKLOAD_FIT_OPTIONS), counting=True)
-    assert abs(flops.mean() - const) / const < 0.01, flops.mean()
+    o, _, flops, _ = oracle.fit(b, abi.default_options(**configs.WORKLOAD_FIT_OPTIONS), counting=True)
+    for name, measured in (("FLOPS_PER_TRACK", flops), ("FLOPS_BWD_PER_TRACK", o.flops_bwd),
+                           ("FLOPS32_PER_TRACK", o.flops32),
+                           ("FLOPS32_BWD_PER_TRACK", o.flops32_bwd)):
+        c = const(name)
+        assert abs(measured.mean() - c) / c < 0.01, (name, measured.mean())

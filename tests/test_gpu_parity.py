@@ -236,3 +236,78 @@ def test_full_size_config2_properties(torch_cuda):
     o, _, _, _ = oracle.fit(sub, c_options(**W))
     g = {k: (v[..., idx] if v.ndim > 1 else v[idx]) for k, v in r1.items()}
     check_parity(g, o)
+
+
+# ---------------------------------------------------------------- mixed mode --
+# FP32 transport Jacobian, covariance transport and material formulas; FP64
+# positions, directions, RK stages, 2x2 inversion, Joseph update and smoother.
+# Bar: every fitted parameter within 1e-5 of the FP64 oracle (metric of
+# tests/parity.py), identical status bits.  The smoothed covariance goes
+# through the RTS cancellation C_f - G (C_p - C_s') G^T, which amplifies the
+# float rounding of the stored filtered covariances; its bar is 1e-3.
+TOL_MIXED_PARAMS = 1e-5
+TOL_MIXED_COV = 1e-3
+
+
+@pytest.mark.parametrize("cfg,n", [(1, 10000), (3, 4000), (5, 4000), (4, 4000)])
+def test_mixed_accuracy_vs_oracle(torch_cuda, cfg, n):
+    b = G.generate(cfg, n_tracks=n)
+    g = run_gpu(b, precision="mixed", **W)
+    o, _, _, _ = oracle.fit(b, c_options(**W))
+    assert np.array_equal(g["status"], o.status), np.unique(g["status"] ^ o.status)
+    assert np.array_equal(g["n_measured"], o.n_measured)
+    pd, cd, c2 = compare_fit(g, o)
+    print(f"config {cfg} mixed max dev (params, cov, chi2): {pd:.2e} {cd:.2e} {c2:.2e}")
+    assert pd <= TOL_MIXED_PARAMS and cd <= TOL_MIXED_COV and c2 <= TOL_MIXED_PARAMS, (pd, cd, c2)
+
+
+def test_mixed_track_states_and_options(torch_cuda):
+    b = G.generate(5, n_tracks=1000)
+    g = run_gpu(b, track_states=True, precision="mixed", **W)
+    o, _, _, _ = oracle.fit(b, c_options(**W), track_states=True)
+    assert np.array_equal(g["status"], o.status)
+    smoothed = ((o.status & abi.KF_STATUS_SMOOTHED) != 0) & ((o.status & abi.KF_STATUS_SMOOTHER_NAN) == 0)
+    # a state's chi2 is a squared residual in units of its sigma: its float
+    # rounding is bounded against max(chi2, 1), not against chi2 itself when
+    # the residual is small
+    worst = compare_states(g["track_states"], o.track_states, o.n_measured, smoothed,
+                           chi2_floor=1.0)
+    print("mixed track-state deviations:", worst)
+    assert worst["geo_id"] == 0
+    for k in ("predicted_par", "filtered_par", "smoothed_par", "path", "chi2"):
+        assert worst[k] <= TOL_MIXED_PARAMS, (k, worst[k])
+    for k in ("predicted_cov", "filtered_cov", "jacobian"):
+        assert worst[k] <= 1e-5, (k, worst[k])
+    assert worst["smoothed_cov"] <= TOL_MIXED_COV, worst["smoothed_cov"]
+    for opt in (dict(smoothing=False), dict(energy_loss=False), dict(max_steps=7)):
+        full = dict(W)
+        full.update(opt)
+        g = run_gpu(b, precision="mixed", **full)
+        o, _, _, _ = oracle.fit(b, c_options(**full))
+        assert np.array_equal(g["status"], o.status), opt
+        pd, cd, c2 = compare_fit(g, o)
+        assert pd <= TOL_MIXED_PARAMS and cd <= TOL_MIXED_COV, (opt, pd, cd)
+
+
+def test_mixed_full_size_config2_pulls(torch_cuda):
+    """1,048,576 tracks in mixed mode: all fitted, pulls N(0, 1), a 10,000-track
+    subset within the error bar of the oracle."""
+    from gpukalmanfitter_amd.fitter import DeviceBatch, KalmanFitter, KalmanFitterOptions
+    b = G.generate(2)
+    kf = KalmanFitter(b.planes, b.field)
+    dev = DeviceBatch(b)
+    kf.fit_batch(dev, KalmanFitterOptions(precision="mixed", **W))
+    r = dev.results_numpy()
+    kf.close()
+    assert ((r["status"] & abi.KF_STATUS_OK) != 0).all()
+    pull = (r["params"] - b.truth) / np.sqrt(r["cov"][abi.DIAG_INDEX])
+    n = b.n_tracks
+    for k in range(6):
+        assert abs(pull[k].mean()) < 5.0 / math.sqrt(n) + 0.01, k
+        assert abs(pull[k].std() - 1.0) < 0.02, k
+    idx = np.sort(np.random.default_rng(11).choice(n, 10000, replace=False))
+    o, _, _, _ = oracle.fit(b.subset(idx), c_options(**W))
+    g = {k: (v[..., idx] if v.ndim > 1 else v[idx]) for k, v in r.items()}
+    assert np.array_equal(g["status"], o.status)
+    pd, cd, c2 = compare_fit(g, o)
+    assert pd <= TOL_MIXED_PARAMS and cd <= TOL_MIXED_COV, (pd, cd)
