@@ -18,7 +18,7 @@
 #include <cstring>
 #include <vector>
 
-#include "../../include/kf_abi.h"
+#include "kf_generate.h"
 
 namespace {
 
@@ -215,37 +215,6 @@ void materialEffects(const GenMaterial &m, double mass, double p, double pathCor
 
 extern "C" {
 
-// Generator configuration.  Planes: plane i of the detector sits at
-// x = first_x + i * pitch with normal +x (local u = y, v = z).
-typedef struct kfg_config {
-  int64_t n_tracks;
-  uint64_t seed;
-  int32_t n_planes;      // planes in the detector
-  int32_t min_planes;    // per-track sequence length ~ U{min_planes..max_planes}
-  int32_t max_planes;
-  int32_t material;      // 1: planes carry silicon of `thickness` (simulated + in the table)
-  double first_x, pitch, half_u, half_v;
-  double thickness;      // mm
-  double hole_prob;      // probability to drop a hit
-  double p_min, p_max, phi_min, phi_max, eta_min, eta_max;
-  double vertex_sigma;   // per axis, mm
-  double hit_sigma_u, hit_sigma_v;
-  double sigma_d0, sigma_z0, sigma_phi, sigma_theta, sigma_p_rel, sigma_t;
-  double mass;
-  // misalignment of the simulated (true) planes, config 4
-  double align_shift_sigma;  // mm, per axis
-  double align_rot_sigma;    // rad, per local axis
-  double true_field_scale;   // field scale used by the simulation
-} kfg_config;
-
-typedef struct kfg_field {
-  int32_t grid;
-  int32_t n[3];
-  double b[3];
-  const float *values;
-  double lo[3];
-  double hi[3];
-} kfg_field;
 
 // Fills the nominal plane table (n_planes entries).
 void kfg_planes(const kfg_config *cfg, kf_plane *planes) {

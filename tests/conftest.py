@@ -18,7 +18,7 @@ def built():
     import subprocess
     if not os.path.exists(os.path.join(ROOT, "oracle", "liboracle.so")):
         subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle")], check=True)
-    for lib in ("libkfgen.so", "libkfmi.so"):
+    for lib in ("libkfgen.so", "libkfmi.so", "kf_run"):
         if not os.path.exists(os.path.join(ROOT, "gpukalmanfitter_amd", lib)):
             subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "gpukalmanfitter_amd", "csrc")],
                            check=True)
