@@ -61,6 +61,8 @@ def parse():
     ap.add_argument("--cpu-sample", type=int, default=1048576,
                     help="tracks of the same workload timed on the CPU oracle")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true",
+                    help="skip the chi2-adjoint leg (config 4, 262,144 tracks)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only copy a MIB-sized device buffer (calibrates rocprofv3 "
                          "FETCH_SIZE / WRITE_SIZE against known bytes) and exit")
@@ -140,6 +142,40 @@ def roofline(prec, n, phase_ms):
                                               (flops[other] - f32[other]) / FP64_PEAK_TFLOPS))}}
 
 
+def adjoint_leg(steps, warmup):
+    """Gradient of the summed chi2 with respect to 60 alignment parameters and
+    the field scale (kf_chi2_gradient) on the alignment workload: 262,144
+    tracks, 10 misaligned planes, field scale 1.003, no material effects."""
+    import numpy as np
+    import torch
+    from gpukalmanfitter_amd import configs
+    from gpukalmanfitter_amd import generate as G
+    from gpukalmanfitter_amd.fitter import DeviceBatch, KalmanFitter, KalmanFitterOptions
+    b = G.generate(4)
+    kf = KalmanFitter(b.planes, b.field)
+    kf.set_field_scale(1.003)
+    dev = DeviceBatch(b)
+    opts = KalmanFitterOptions(**dict(configs.WORKLOAD_FIT_OPTIONS, multiple_scattering=False,
+                                      energy_loss=False))
+    for _ in range(warmup):
+        kf.chi2_gradient(dev, opts, synchronize=False)
+    torch.cuda.synchronize()
+    kms = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        kf.chi2_gradient(dev, opts, synchronize=False)
+        kms.append(kf.last_kernel_ms())
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    _, _, _, flags, _ = kf.chi2_gradient(dev, opts)
+    kf.close()
+    return {"workload": "config4-alignment: 262144 tracks, 10 misaligned x-normal planes, Bz=2 T "
+                        "x 1.003, d(sum chi2)/d(60 alignment params + field scale)",
+            "tracks_per_s": b.n_tracks * steps / el, "ms_per_step": el * 1e3 / steps,
+            "kernel_ms": float(np.mean(kms)), "tracks_left_out": int((flags != 0).sum()),
+            "dtype": "f64"}
+
+
 def main():
     args = parse()
     if args.copy_calibration:
@@ -204,6 +240,11 @@ def main():
 
     head = precisions[0]
     roof = roofline(head, n, phase_ms[head]) if args.config == 2 else None
+    adj = None
+    if not args.no_adjoint:
+        adj = adjoint_leg(max(args.steps // 2, 3), 1)
+        print(f"[bench] adjoint: {adj['tracks_per_s']:.4g} tracks/s, {adj['ms_per_step']:.3f} ms/step",
+              file=sys.stderr, flush=True)
     cpu = None
     errors = None
     if not args.no_cpu:
@@ -234,6 +275,7 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
         "error_vs_oracle": errors,
+        "adjoint": adj,
     }
     if "fp64" in rates and "mixed" in rates:
         out["mixed_over_fp64"] = rates["mixed"] / rates["fp64"]

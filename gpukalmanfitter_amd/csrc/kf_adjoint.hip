@@ -1,0 +1,105 @@
+// kf_adjoint.hip -- kernels of the chi2 adjoint (kf_adjoint.hpp) and their
+// launcher: the reverse sweep per track, a deterministic reduction over
+// tracks, and the conversion to per-plane alignment parameters.
+#include <hip/hip_runtime.h>
+
+#include "kf_adjoint.hpp"
+
+namespace kfd {
+
+__global__ void __launch_bounds__(KF_BLOCK) kf_adjoint_kernel(FitArgs args, AdjArgs adj) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t chunk = __builtin_amdgcn_readfirstlane((uint32_t)(i >> 6));
+  double *ck = args.tsc + chunk * (size_t)args.max_states * KF_CK_N * 64u;
+  double *stk = adj.stack + chunk * (size_t)adj.max_seg_steps * kSR_N * 64u;
+  if (i >= args.n) return;
+  TrackAdjoint ta(args, adj, i, ck, stk, (uint32_t)(i & 63));
+  ta.run();
+}
+
+// Block p < n_planes sums the 12 plane adjoints of plane p over all tracks;
+// block n_planes the field-scale adjoint; block n_planes + 1 the chi2.  Each
+// thread sums a fixed stride of tracks, then a fixed-order tree: the result
+// does not depend on scheduling.
+constexpr int kRedThreads = 256;
+__global__ void __launch_bounds__(kRedThreads) kf_adjoint_reduce(FitArgs args, AdjArgs adj,
+                                                                 double *sums) {
+  __shared__ double red[kRedThreads][kPG_N];
+  const int p = blockIdx.x;
+  double acc[kPG_N];
+#pragma unroll
+  for (int f = 0; f < kPG_N; ++f) acc[f] = 0.0;
+  for (int64_t i = threadIdx.x; i < args.n; i += kRedThreads) {
+    if (p < args.n_planes) {
+      const int q = p - args.seq_first[i];
+      if (q >= 0 && q < args.seq_count[i] && q < adj.max_seq) {
+#pragma unroll
+        for (int f = 0; f < kPG_N; ++f) acc[f] += adj.plane_adj[((size_t)q * kPG_N + f) * args.n + i];
+      }
+    } else if (p == args.n_planes) {
+      acc[0] += adj.scale_adj[i];
+    } else {
+      acc[0] += args.out_chi2[i];
+    }
+  }
+#pragma unroll
+  for (int f = 0; f < kPG_N; ++f) red[threadIdx.x][f] = acc[f];
+  __syncthreads();
+  for (int w = kRedThreads / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+#pragma unroll
+      for (int f = 0; f < kPG_N; ++f) red[threadIdx.x][f] += red[threadIdx.x + w][f];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x < kPG_N) sums[(size_t)p * kPG_N + threadIdx.x] = red[0][threadIdx.x];
+}
+
+// dX/d(du, dv, dw) = R^T dX/dc; dX/d(a, b, c) = <dX/dR, R G> with G the
+// generators of Rx, Ry, Rz at zero (R' = R Rx(a) Ry(b) Rz(c)).
+__global__ void kf_adjoint_finish(FitArgs args, const double *sums, double *grad_planes,
+                                  double *grad_scale, double *chi2) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p < args.n_planes) {
+    const double *s = sums + (size_t)p * kPG_N;
+    const double *r = args.planes[p].r;  // row-major R[k][j] = r[3k + j]
+    double g[6];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) g[j] = r[j] * s[0] + r[3 + j] * s[1] + r[6 + j] * s[2];
+    const double *Rb = s + kPG_R;
+    // <Rb, R Gx> = Rb[:,1] . R[:,2] - Rb[:,2] . R[:,1]
+    double gx = 0.0, gy = 0.0, gz = 0.0;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      gx += Rb[3 * k + 1] * r[3 * k + 2] - Rb[3 * k + 2] * r[3 * k + 1];
+      gy += Rb[3 * k + 2] * r[3 * k + 0] - Rb[3 * k + 0] * r[3 * k + 2];
+      gz += Rb[3 * k + 0] * r[3 * k + 1] - Rb[3 * k + 1] * r[3 * k + 0];
+    }
+    g[3] = gx;
+    g[4] = gy;
+    g[5] = gz;
+#pragma unroll
+    for (int j = 0; j < 6; ++j) grad_planes[(size_t)p * 6 + j] = g[j];
+  }
+  if (p == 0) {
+    if (grad_scale) grad_scale[0] = sums[(size_t)args.n_planes * kPG_N];
+    if (chi2) chi2[0] = sums[(size_t)(args.n_planes + 1) * kPG_N];
+  }
+}
+
+}  // namespace kfd
+
+extern "C" hipError_t kfd_launch_adjoint(const kfd::FitArgs *args, const kfd::AdjArgs *adj,
+                                         double *sums, double *grad_planes, double *grad_scale,
+                                         double *chi2, hipStream_t stream) {
+  const int64_t blocks = (args->n + KF_BLOCK - 1) / KF_BLOCK;
+  if (blocks > 0)
+    hipLaunchKernelGGL(kfd::kf_adjoint_kernel, dim3((unsigned)blocks), dim3(KF_BLOCK), 0, stream,
+                       *args, *adj);
+  hipLaunchKernelGGL(kfd::kf_adjoint_reduce, dim3((unsigned)(args->n_planes + 2)),
+                     dim3(kfd::kRedThreads), 0, stream, *args, *adj, sums);
+  hipLaunchKernelGGL(kfd::kf_adjoint_finish, dim3((unsigned)((args->n_planes + 63) / 64)),
+                     dim3(64), 0, stream, *args, (const double *)sums, grad_planes, grad_scale,
+                     chi2);
+  return hipGetLastError();
+}

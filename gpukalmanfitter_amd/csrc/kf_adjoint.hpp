@@ -1,0 +1,1056 @@
+// kf_adjoint.hpp -- hand-written adjoint of the summed fit chi2.
+//
+// Objective: X = sum over tracks and measured surfaces of the predicted-
+// residual chi2 r^T S^-1 r of the forward filter (the fit's results.chi2).
+// Parameters: per plane, the centre c and the rotation R (columns U, V,
+// normal), and the field scale s.  The reduction kernel maps (dX/dc, dX/dR)
+// to the six alignment parameters of a plane (local shifts and small local
+// rotations, R' = R Rx(a) Ry(b) Rz(c), c' = c + R (du, dv, dw)).
+//
+// One track per lane, reverse mode by hand.  The forward filter (TrackFit)
+// runs first and writes a checkpoint of the state after the update at every
+// measured surface.  The reverse sweep then walks the measured surfaces from
+// the last to the first; for each it reloads the checkpoint of the surface
+// before, recomputes that segment's RK steps (their inputs go to a per-lane
+// stack in global memory; the RK stages themselves are recomputed again in
+// the backward step), recomputes the bound state, chi2 and gain update at the
+// surface, and back-propagates:
+//   update + chi2 -> bound state (parameters, covariance transport
+//   jb C jb^T with the path-length correction) -> RK steps in reverse (stage
+//   vectors, transport matrix D and J <- D J) -> the checkpoint state.
+// Only the spatial 5x5 blocks enter: the time row and column of the
+// covariance and Jacobians never reach the chi2.
+//
+// Derivative conventions: discrete decisions (step accept/reject, surface
+// reached, hole) are held fixed; a step whose length was set by the
+// navigator (distance to the target plane) is differentiated through that
+// distance, a step set by the error control is held fixed.  Material
+// interaction is not differentiated (the entry point refuses material).
+#pragma once
+
+#include "kf_trackfit.hpp"
+
+namespace kfd {
+
+// per-lane stack of the RK steps of the segment being reversed
+enum {
+  kSR_POS = 0,   // position before the step (3)
+  kSR_DIR = 3,   // direction before the step (3)
+  kSR_H = 6,     // accepted step length
+  kSR_G = 7,     // J direction rows (9) before the step
+  kSR_TGT = 16,  // target plane index + 1 if the step length was the navigator's, else 0
+  kSR_N = 17
+};
+
+// per-track plane adjoints: slot q (sequence position) holds dX/dc (3) and
+// dX/dR (9, row-major) of plane seq_first + q
+enum { kPG_C = 0, kPG_R = 3, kPG_N = 12 };
+
+struct AdjArgs {
+  double *stack;          // [chunk][max_seg_steps][kSR_N][64]
+  int32_t max_seg_steps;  // stack depth per lane
+  int32_t max_seq;        // plane-adjoint slots per track (longest sequence)
+  double *plane_adj;      // [max_seq][kPG_N][n]
+  double *scale_adj;      // [n]
+  uint32_t *status;       // [n]: 0 ok, else KF_ADJ_* bits
+};
+
+
+// a x b
+KFD_INLINE void crs(const double a[3], const double b[3], double r[3]) {
+  r[0] = a[1] * b[2] - a[2] * b[1];
+  r[1] = a[2] * b[0] - a[0] * b[2];
+  r[2] = a[0] * b[1] - a[1] * b[0];
+}
+// c = a x b:  abar += b x cbar,  bbar += cbar x a
+KFD_INLINE void crsVJP(const double a[3], const double b[3], const double cb[3], double ab[3],
+                       double bb[3]) {
+  double t[3];
+  crs(b, cb, t);
+#pragma unroll
+  for (int k = 0; k < 3; ++k) ab[k] += t[k];
+  crs(cb, a, t);
+#pragma unroll
+  for (int k = 0; k < 3; ++k) bb[k] += t[k];
+}
+// M(B) v = v x B, as a matrix
+KFD_INLINE void crossMat(const double B[3], double M[3][3]) {
+  M[0][0] = 0.0;
+  M[0][1] = B[2];
+  M[0][2] = -B[1];
+  M[1][0] = -B[2];
+  M[1][1] = 0.0;
+  M[1][2] = B[0];
+  M[2][0] = B[1];
+  M[2][1] = -B[0];
+  M[2][2] = 0.0;
+}
+// adjoint of B from the adjoint of M(B)
+KFD_INLINE void crossMatVJP(const double Mb[3][3], double Bb[3]) {
+  Bb[0] += Mb[1][2] - Mb[2][1];
+  Bb[1] += Mb[2][0] - Mb[0][2];
+  Bb[2] += Mb[0][1] - Mb[1][0];
+}
+
+// T = qop M X, X = I + w Tp.  Tb in; accumulates Bb, qb, Tpb, hb (dX/dh = dwdh Tp).
+KFD_INLINE void chainVJP(const double M[3][3], const double X[3][3], const double T[3][3],
+                         double qop, const double Tb[3][3], double Bb[3], double &qb,
+                         double Tpb[3][3], double w, const double Tp[3][3], double &hb,
+                         double dwdh) {
+  double Mb[3][3], Xb[3][3];
+#pragma unroll
+  for (int r = 0; r < 3; ++r)
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      double sM = 0.0, sX = 0.0;
+#pragma unroll
+      for (int m = 0; m < 3; ++m) {
+        sM += Tb[r][m] * X[q][m];  // (Tb X^T)[r][q]
+        sX += M[m][r] * Tb[m][q];  // (M^T Tb)[r][q]
+      }
+      Mb[r][q] = qop * sM;
+      Xb[r][q] = qop * sX;
+    }
+  // qop-derivative: T = qop (M X) -> qb += <Tb, M X>
+#pragma unroll
+  for (int r = 0; r < 3; ++r)
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      double s = 0.0;
+#pragma unroll
+      for (int m = 0; m < 3; ++m) s += M[r][m] * X[m][q];
+      qb += Tb[r][q] * s;
+    }
+  crossMatVJP(Mb, Bb);
+#pragma unroll
+  for (int r = 0; r < 3; ++r)
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      Tpb[r][q] += w * Xb[r][q];
+      hb += dwdh * Xb[r][q] * Tp[r][q];
+    }
+}
+
+// B = s F(x): sb += Bb . F(x) = Bb . B / s; xb += (dB/dx)^T Bb on a grid.
+KFD_INLINE void fieldVJP(const DevField &fld, const double x[3], const double B[3],
+                         const double Bb[3], double xb[3], double &sb) {
+  sb += (Bb[0] * B[0] + Bb[1] * B[1] + Bb[2] * B[2]) / fld.scale;
+  if (fld.grid) {
+    double Bg[3], dB[3][3];
+    fieldAtGrad(fld, x, Bg, dB);
+#pragma unroll
+    for (int a = 0; a < 3; ++a) xb[a] += Bb[0] * dB[0][a] + Bb[1] * dB[1][a] + Bb[2] * dB[2][a];
+  }
+}
+
+// One RK4 step (EigenStepper.ipp:226-332 at its accepted length) with every
+// intermediate the backward step needs.
+struct StepFwd {
+  double B1[3], Bm[3], Bl[3];
+  double k1[3], k2[3], k3[3], k4[3];
+  double t2[3], t3[3], t4[3];
+  double p1[3], p2[3];
+  double L1[3], L2[3], L3[3], L4[3];
+  double T1[3][3], T2[3][3], T3[3][3], T4[3][3];
+  double X2[3][3], X3[3][3], X4[3][3];
+  double dFdT[3][3], dGdT[3][3], dFdL[3], dGdL[3];
+  double du[3], dun;
+
+  KFD_INLINE void run(const DevField &fld, const double pos[3], const double d[3], double h,
+                      double qop) {
+    const double hh = h * 0.5;
+    fieldAt(fld, pos[0], pos[1], pos[2], B1);
+    crs(d, B1, k1);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) k1[k] *= qop;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) p1[k] = pos[k] + hh * d[k] + (h * h * 0.125) * k1[k];
+    fieldAt(fld, p1[0], p1[1], p1[2], Bm);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) t2[k] = d[k] + hh * k1[k];
+    crs(t2, Bm, k2);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) k2[k] *= qop;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) t3[k] = d[k] + hh * k2[k];
+    crs(t3, Bm, k3);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) k3[k] *= qop;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) p2[k] = pos[k] + h * d[k] + (h * h * 0.5) * k3[k];
+    fieldAt(fld, p2[0], p2[1], p2[2], Bl);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) t4[k] = d[k] + h * k3[k];
+    crs(t4, Bl, k4);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) k4[k] *= qop;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) du[k] = d[k] + (h / 6.0) * (k1[k] + 2.0 * (k2[k] + k3[k]) + k4[k]);
+    dun = sqrt(du[0] * du[0] + du[1] * du[1] + du[2] * du[2]);
+    // transport matrix blocks (EigenStepper.ipp:50-124)
+    double c[3];
+    crs(d, B1, L1);
+    crs(t2, Bm, L2);
+    crs(L1, Bm, c);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) L2[k] += (qop * hh) * c[k];
+    crs(t3, Bm, L3);
+    crs(L2, Bm, c);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) L3[k] += (qop * hh) * c[k];
+    crs(t4, Bl, L4);
+    crs(L3, Bl, c);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) L4[k] += (qop * h) * c[k];
+    double M1[3][3], Mm[3][3], Ml[3][3];
+    crossMat(B1, M1);
+    crossMat(Bm, Mm);
+    crossMat(Bl, Ml);
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+      for (int q = 0; q < 3; ++q) T1[r][q] = qop * M1[r][q];
+    chain(Mm, T1, hh, qop, X2, T2);
+    chain(Mm, T2, hh, qop, X3, T3);
+    chain(Ml, T3, h, qop, X4, T4);
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+#pragma unroll
+      for (int q = 0; q < 3; ++q) {
+        const double id = (r == q) ? 1.0 : 0.0;
+        dFdT[r][q] = h * (id + (h / 6.0) * (T1[r][q] + T2[r][q] + T3[r][q]));
+        dGdT[r][q] = id + (h / 6.0) * (T1[r][q] + 2.0 * (T2[r][q] + T3[r][q]) + T4[r][q]);
+      }
+      dFdL[r] = (h * h / 6.0) * (L1[r] + L2[r] + L3[r]);
+      dGdL[r] = (h / 6.0) * (L1[r] + 2.0 * (L2[r] + L3[r]) + L4[r]);
+    }
+  }
+  // X = I + w Tprev, T = qop M X
+  KFD_INLINE static void chain(const double M[3][3], const double Tp[3][3], double w, double qop,
+                               double X[3][3], double T[3][3]) {
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+      for (int q = 0; q < 3; ++q) X[r][q] = ((r == q) ? 1.0 : 0.0) + w * Tp[r][q];
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+      for (int q = 0; q < 3; ++q) {
+        double s = 0.0;
+#pragma unroll
+        for (int m = 0; m < 3; ++m) s += M[r][m] * X[m][q];
+        T[r][q] = qop * s;
+      }
+  }
+};
+
+// Backward of one RK step.  In: the adjoints of the step's outputs (position
+// pb, direction db, Jacobian Ab/Gb, and k4b for the propagation derivative of
+// the last step).  Out (in place): the adjoints of its inputs (position,
+// direction, Jacobian), and accumulated hb, qb (q/p) and sb (field scale).
+KFD_INLINE void stepVJP(const DevField &fld, const StepFwd &F, const double pos[3],
+                        const double d[3], double h, double qop, const double Gin[3][3],
+                        double pb[3], double db[3], double Gb[3][3], const double Ab[3][5],
+                        const double k4bIn[3], double &hb, double &qb, double &sb) {
+  const double hh = h * 0.5;
+  double k1b[3] = {0, 0, 0}, k2b[3] = {0, 0, 0}, k3b[3] = {0, 0, 0};
+  double k4b[3] = {k4bIn[0], k4bIn[1], k4bIn[2]};
+  double t2b[3] = {0, 0, 0}, t3b[3] = {0, 0, 0}, t4b[3] = {0, 0, 0};
+  double B1b[3] = {0, 0, 0}, Bmb[3] = {0, 0, 0}, Blb[3] = {0, 0, 0};
+  double dInb[3] = {0, 0, 0}, posInb[3] = {pb[0], pb[1], pb[2]};
+  // ---- J <- D J:  A[:,2+c] += dFdT G[:,c] (+ dFdL for c = 2), G[:,c] = dGdT G[:,c] (+ dGdL)
+  double dFdTb[3][3], dGdTb[3][3], dFdLb[3], dGdLb[3], GinB[3][3];
+#pragma unroll
+  for (int r = 0; r < 3; ++r) {
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      double sF = 0.0, sG = 0.0;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        sF += Ab[r][2 + c] * Gin[q][c];
+        sG += Gb[r][c] * Gin[q][c];
+      }
+      dFdTb[r][q] = sF;
+      dGdTb[r][q] = sG;
+    }
+    dFdLb[r] = Ab[r][4];
+    dGdLb[r] = Gb[r][2];
+  }
+#pragma unroll
+  for (int q = 0; q < 3; ++q)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      double s = 0.0;
+#pragma unroll
+      for (int r = 0; r < 3; ++r) s += F.dFdT[r][q] * Ab[r][2 + c] + F.dGdT[r][q] * Gb[r][c];
+      GinB[q][c] = s;
+    }
+  // ---- dFdT = h I + h^2/6 (T1+T2+T3), dGdT = I + h/6 (T1 + 2T2 + 2T3 + T4), dFdL, dGdL
+  double T1b[3][3], T2b[3][3], T3b[3][3], T4b[3][3];
+#pragma unroll
+  for (int r = 0; r < 3; ++r)
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      const double sT = F.T1[r][q] + F.T2[r][q] + F.T3[r][q];
+      const double sG = F.T1[r][q] + 2.0 * (F.T2[r][q] + F.T3[r][q]) + F.T4[r][q];
+      hb += dFdTb[r][q] * (((r == q) ? 1.0 : 0.0) + (h / 3.0) * sT) + dGdTb[r][q] * (sG / 6.0);
+      const double a = (h * h / 6.0) * dFdTb[r][q], g = (h / 6.0) * dGdTb[r][q];
+      T1b[r][q] = a + g;
+      T2b[r][q] = a + 2.0 * g;
+      T3b[r][q] = a + 2.0 * g;
+      T4b[r][q] = g;
+    }
+  double L1b[3], L2b[3], L3b[3], L4b[3];
+#pragma unroll
+  for (int r = 0; r < 3; ++r) {
+    hb += dFdLb[r] * (h / 3.0) * (F.L1[r] + F.L2[r] + F.L3[r]) +
+          dGdLb[r] * (F.L1[r] + 2.0 * (F.L2[r] + F.L3[r]) + F.L4[r]) / 6.0;
+    const double a = (h * h / 6.0) * dFdLb[r], g = (h / 6.0) * dGdLb[r];
+    L1b[r] = a + g;
+    L2b[r] = a + 2.0 * g;
+    L3b[r] = a + 2.0 * g;
+    L4b[r] = g;
+  }
+  // ---- T chain: T4 = qop M(Bl) X4, X4 = I + h T3; T3, T2 with M(Bm) and hh; T1 = qop M(B1)
+  {
+    double Ml[3][3], Mm[3][3], M1[3][3];
+    crossMat(F.Bl, Ml);
+    crossMat(F.Bm, Mm);
+    crossMat(F.B1, M1);
+    chainVJP(Ml, F.X4, F.T4, qop, T4b, Blb, qb, T3b, h, F.T3, hb, 1.0);
+    chainVJP(Mm, F.X3, F.T3, qop, T3b, Bmb, qb, T2b, hh, F.T2, hb, 0.5);
+    chainVJP(Mm, F.X2, F.T2, qop, T2b, Bmb, qb, T1b, hh, F.T1, hb, 0.5);
+    double Mb[3][3];
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+      for (int q = 0; q < 3; ++q) {
+        qb += T1b[r][q] * M1[r][q];
+        Mb[r][q] = qop * T1b[r][q];
+      }
+    crossMatVJP(Mb, B1b);
+  }
+  // ---- L chain: L4 = t4 x Bl + (qop h)(L3 x Bl); L3, L2 with Bm and hh; L1 = d x B1
+  {
+    double c[3];
+    crsVJP(F.t4, F.Bl, L4b, t4b, Blb);
+    crs(F.L3, F.Bl, c);
+    qb += h * (L4b[0] * c[0] + L4b[1] * c[1] + L4b[2] * c[2]);
+    hb += qop * (L4b[0] * c[0] + L4b[1] * c[1] + L4b[2] * c[2]);
+    double w[3] = {qop * h * L4b[0], qop * h * L4b[1], qop * h * L4b[2]};
+    crsVJP(F.L3, F.Bl, w, L3b, Blb);
+
+    crsVJP(F.t3, F.Bm, L3b, t3b, Bmb);
+    crs(F.L2, F.Bm, c);
+    qb += hh * (L3b[0] * c[0] + L3b[1] * c[1] + L3b[2] * c[2]);
+    hb += 0.5 * qop * (L3b[0] * c[0] + L3b[1] * c[1] + L3b[2] * c[2]);
+    double w3[3] = {qop * hh * L3b[0], qop * hh * L3b[1], qop * hh * L3b[2]};
+    crsVJP(F.L2, F.Bm, w3, L2b, Bmb);
+
+    crsVJP(F.t2, F.Bm, L2b, t2b, Bmb);
+    crs(F.L1, F.Bm, c);
+    qb += hh * (L2b[0] * c[0] + L2b[1] * c[1] + L2b[2] * c[2]);
+    hb += 0.5 * qop * (L2b[0] * c[0] + L2b[1] * c[1] + L2b[2] * c[2]);
+    double w2[3] = {qop * hh * L2b[0], qop * hh * L2b[1], qop * hh * L2b[2]};
+    crsVJP(F.L1, F.Bm, w2, L1b, Bmb);
+
+    crsVJP(d, F.B1, L1b, dInb, B1b);
+  }
+  // ---- outputs: pos' = pos + h d + h^2/6 (k1+k2+k3); d' = du / |du|
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    dInb[k] += h * pb[k];
+    hb += pb[k] * (d[k] + (h / 3.0) * (F.k1[k] + F.k2[k] + F.k3[k]));
+    k1b[k] += (h * h / 6.0) * pb[k];
+    k2b[k] += (h * h / 6.0) * pb[k];
+    k3b[k] += (h * h / 6.0) * pb[k];
+  }
+  {
+    double dn[3] = {F.du[0] / F.dun, F.du[1] / F.dun, F.du[2] / F.dun};
+    const double pr = dn[0] * db[0] + dn[1] * db[1] + dn[2] * db[2];
+    double dub[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) dub[k] = (db[k] - dn[k] * pr) / F.dun;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      dInb[k] += dub[k];
+      hb += dub[k] * (F.k1[k] + 2.0 * (F.k2[k] + F.k3[k]) + F.k4[k]) / 6.0;
+      k1b[k] += (h / 6.0) * dub[k];
+      k2b[k] += (h / 3.0) * dub[k];
+      k3b[k] += (h / 3.0) * dub[k];
+      k4b[k] += (h / 6.0) * dub[k];
+    }
+  }
+  // ---- stages, last to first
+  double c[3], w[3];
+  // k4 = qop (t4 x Bl)
+  crs(F.t4, F.Bl, c);
+  qb += k4b[0] * c[0] + k4b[1] * c[1] + k4b[2] * c[2];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) w[k] = qop * k4b[k];
+  crsVJP(F.t4, F.Bl, w, t4b, Blb);
+  // t4 = d + h k3
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    dInb[k] += t4b[k];
+    hb += t4b[k] * F.k3[k];
+    k3b[k] += h * t4b[k];
+  }
+  // Bl = s F(p2)
+  double p2b[3] = {0, 0, 0};
+  fieldVJP(fld, F.p2, F.Bl, Blb, p2b, sb);
+  // p2 = pos + h d + h^2/2 k3
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    posInb[k] += p2b[k];
+    dInb[k] += h * p2b[k];
+    hb += p2b[k] * (d[k] + h * F.k3[k]);
+    k3b[k] += (h * h * 0.5) * p2b[k];
+  }
+  // k3 = qop (t3 x Bm)
+  crs(F.t3, F.Bm, c);
+  qb += k3b[0] * c[0] + k3b[1] * c[1] + k3b[2] * c[2];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) w[k] = qop * k3b[k];
+  crsVJP(F.t3, F.Bm, w, t3b, Bmb);
+  // t3 = d + hh k2
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    dInb[k] += t3b[k];
+    hb += 0.5 * t3b[k] * F.k2[k];
+    k2b[k] += hh * t3b[k];
+  }
+  // k2 = qop (t2 x Bm)
+  crs(F.t2, F.Bm, c);
+  qb += k2b[0] * c[0] + k2b[1] * c[1] + k2b[2] * c[2];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) w[k] = qop * k2b[k];
+  crsVJP(F.t2, F.Bm, w, t2b, Bmb);
+  // t2 = d + hh k1
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    dInb[k] += t2b[k];
+    hb += 0.5 * t2b[k] * F.k1[k];
+    k1b[k] += hh * t2b[k];
+  }
+  // Bm = s F(p1)
+  double p1b[3] = {0, 0, 0};
+  fieldVJP(fld, F.p1, F.Bm, Bmb, p1b, sb);
+  // p1 = pos + hh d + h^2/8 k1
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    posInb[k] += p1b[k];
+    dInb[k] += hh * p1b[k];
+    hb += p1b[k] * (0.5 * d[k] + 0.25 * h * F.k1[k]);
+    k1b[k] += (h * h * 0.125) * p1b[k];
+  }
+  // k1 = qop (d x B1)
+  crs(d, F.B1, c);
+  qb += k1b[0] * c[0] + k1b[1] * c[1] + k1b[2] * c[2];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) w[k] = qop * k1b[k];
+  crsVJP(d, F.B1, w, dInb, B1b);
+  // B1 = s F(pos)
+  fieldVJP(fld, pos, F.B1, B1b, posInb, sb);
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    pb[k] = posInb[k];
+    db[k] = dInb[k];
+  }
+#pragma unroll
+  for (int q = 0; q < 3; ++q)
+#pragma unroll
+    for (int cc = 0; cc < 3; ++cc) Gb[q][cc] = GinB[q][cc];
+}
+
+// ---------------------------------------------------------------------------
+// Bound state on a plane and gain-matrix update, spatial 5x5 part, with the
+// intermediates their backward pass needs (EigenStepper.ipp:516-540,
+// CovarianceEngine.hpp:269-290, GainMatrixUpdater.hpp:34-106).
+struct PlaneFwd {
+  double u0[3], u1[3], nr[3], cc[3];
+  double rho, l20, l21, l30, l31, l32;
+  double dp, nv[3], sv[5];
+  double jpP[3][5], jpD[3][5];
+  double jb[5][5];
+  double Cm[5][5], Cp[5][5];
+  double xp[5];
+  double V00, V01, V11, Si00, Si01, Si11, r[2], u[2], xf[5], chi2;
+
+  // d, pos: direction and position at the plane; A, G: J at the plane; k4:
+  // propagation derivative of the last step; Cm: covariance at the segment
+  // start; m/V: the hit.
+  KFD_INLINE void run(const DevPlane &pl, const double d[3], const double pos[3],
+                      const double A[3][5], const double G[3][3], const double k4[3],
+                      double qop, const double cm[21], double m0, double m1, double v00,
+                      double v01, double v11) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      u0[k] = pl.r[3 * k];
+      u1[k] = pl.r[3 * k + 1];
+      nr[k] = pl.r[3 * k + 2];
+      cc[k] = pl.c[k];
+    }
+    rho = sqrt(d[0] * d[0] + d[1] * d[1]);
+    const double cosT = d[2], sinT = rho, invSinT = 1.0 / rho;
+    const double cosP = d[0] * invSinT, sinP = d[1] * invSinT;
+    l20 = -sinP * invSinT;
+    l21 = cosP * invSinT;
+    l30 = cosP * cosT;
+    l31 = sinP * cosT;
+    l32 = -sinT;
+    dp = nr[0] * d[0] + nr[1] * d[1] + nr[2] * d[2];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) nv[k] = nr[k] / dp;
+#pragma unroll
+    for (int c = 0; c < 5; ++c) {
+      sv[c] = nv[0] * A[0][c] + nv[1] * A[1][c] + nv[2] * A[2][c];
+#pragma unroll
+      for (int q = 0; q < 3; ++q) {
+        jpP[q][c] = A[q][c] - d[q] * sv[c];
+        jpD[q][c] = ((c >= 2) ? G[q][c - 2] : 0.0) - k4[q] * sv[c];
+      }
+      jb[0][c] = u0[0] * jpP[0][c] + u0[1] * jpP[1][c] + u0[2] * jpP[2][c];
+      jb[1][c] = u1[0] * jpP[0][c] + u1[1] * jpP[1][c] + u1[2] * jpP[2][c];
+      jb[2][c] = l20 * jpD[0][c] + l21 * jpD[1][c];
+      jb[3][c] = l30 * jpD[0][c] + l31 * jpD[1][c] + l32 * jpD[2][c];
+      jb[4][c] = (c == 4) ? 1.0 : 0.0;
+    }
+#pragma unroll
+    for (int q = 0; q < 5; ++q)
+#pragma unroll
+      for (int c = 0; c < 5; ++c) Cm[q][c] = cm[sidx(q, c)];
+    double T[5][5];  // jb Cm
+#pragma unroll
+    for (int q = 0; q < 5; ++q)
+#pragma unroll
+      for (int c = 0; c < 5; ++c) {
+        double s = 0.0;
+#pragma unroll
+        for (int m = 0; m < 5; ++m) s += jb[q][m] * Cm[m][c];
+        T[q][c] = s;
+      }
+#pragma unroll
+    for (int q = 0; q < 5; ++q)
+#pragma unroll
+      for (int c = 0; c < 5; ++c) {
+        double s = 0.0;
+#pragma unroll
+        for (int m = 0; m < 5; ++m) s += T[q][m] * jb[c][m];
+        Cp[q][c] = s;
+      }
+    xp[0] = u0[0] * (pos[0] - cc[0]) + u0[1] * (pos[1] - cc[1]) + u0[2] * (pos[2] - cc[2]);
+    xp[1] = u1[0] * (pos[0] - cc[0]) + u1[1] * (pos[1] - cc[1]) + u1[2] * (pos[2] - cc[2]);
+    xp[2] = atan2(d[1], d[0]);
+    xp[3] = atan2(rho, d[2]);
+    xp[4] = qop;
+    // gain-matrix update
+    V00 = v00;
+    V01 = v01;
+    V11 = v11;
+    const double S00 = Cp[0][0] + V00, S01 = Cp[0][1] + V01, S11 = Cp[1][1] + V11;
+    const double det = S00 * S11 - S01 * S01;
+    Si00 = S11 / det;
+    Si01 = -S01 / det;
+    Si11 = S00 / det;
+    r[0] = m0 - xp[0];
+    r[1] = m1 - xp[1];
+    u[0] = Si00 * r[0] + Si01 * r[1];
+    u[1] = Si01 * r[0] + Si11 * r[1];
+    chi2 = r[0] * u[0] + r[1] * u[1];
+#pragma unroll
+    for (int q = 0; q < 5; ++q) xf[q] = xp[q] + Cp[q][0] * u[0] + Cp[q][1] * u[1];
+  }
+
+  // Backward of chi2 + update + bound state.  In: xfb (adjoint of the
+  // filtered parameters), Cfb (symmetric adjoint of the filtered covariance),
+  // chi2 weight 1.  Out: the adjoints of pos, d, A, G, k4, q/p and of the
+  // segment-start covariance (Cmb, symmetric), and of the plane (cb, Rb).
+  KFD_INLINE void vjp(const double d[3], const double pos[3], const double A[3][5],
+                      const double k4[3], const double xfb[5], const double Cfb[5][5],
+                      double posb[3], double db[3], double Ab[3][5], double Gb[3][3],
+                      double k4b[3], double &qb, double Cmb[5][5], double cb[3],
+                      double Rb[3][3]) const {
+    // ---- update: xf = xp + W u, Cf = Cp - W Si W^T, chi2 = r^T u, u = Si r, r = m - H xp
+    double xpb[5], Cpb[5][5];
+    const double W[5][2] = {{Cp[0][0], Cp[0][1]}, {Cp[1][0], Cp[1][1]}, {Cp[2][0], Cp[2][1]},
+                            {Cp[3][0], Cp[3][1]}, {Cp[4][0], Cp[4][1]}};
+    const double Si[2][2] = {{Si00, Si01}, {Si01, Si11}};
+    double Wb[5][2], ub[2] = {0, 0}, rb[2], Sib[2][2];
+#pragma unroll
+    for (int q = 0; q < 5; ++q) {
+      xpb[q] = xfb[q];
+      Wb[q][0] = xfb[q] * u[0];
+      Wb[q][1] = xfb[q] * u[1];
+      ub[0] += W[q][0] * xfb[q];
+      ub[1] += W[q][1] * xfb[q];
+#pragma unroll
+      for (int c = 0; c < 5; ++c) Cpb[q][c] = Cfb[q][c];
+    }
+    // chi2 = r^T Si r (weight 1)
+    rb[0] = 2.0 * u[0];
+    rb[1] = 2.0 * u[1];
+#pragma unroll
+    for (int a2 = 0; a2 < 2; ++a2)
+#pragma unroll
+      for (int b2 = 0; b2 < 2; ++b2) Sib[a2][b2] = r[a2] * r[b2] + ub[a2] * r[b2];
+    rb[0] += Si[0][0] * ub[0] + Si[0][1] * ub[1];
+    rb[1] += Si[1][0] * ub[0] + Si[1][1] * ub[1];
+    // Cf term: Wb -= 2 Cfb W Si, Sib -= W^T Cfb W
+    {
+      double CW[5][2];
+#pragma unroll
+      for (int q = 0; q < 5; ++q)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+          double s = 0.0;
+#pragma unroll
+          for (int m = 0; m < 5; ++m) s += Cfb[q][m] * W[m][j];
+          CW[q][j] = s;
+        }
+#pragma unroll
+      for (int q = 0; q < 5; ++q)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) Wb[q][j] -= 2.0 * (CW[q][0] * Si[0][j] + CW[q][1] * Si[1][j]);
+#pragma unroll
+      for (int a2 = 0; a2 < 2; ++a2)
+#pragma unroll
+        for (int b2 = 0; b2 < 2; ++b2) {
+          double s = 0.0;
+#pragma unroll
+          for (int m = 0; m < 5; ++m) s += W[m][a2] * CW[m][b2];
+          Sib[a2][b2] -= s;
+        }
+    }
+    // Si = S^-1: Sb = -Si Sib Si, symmetrised; S = Cp[0:2,0:2] + V
+    {
+      double t[2][2], Sb[2][2];
+#pragma unroll
+      for (int a2 = 0; a2 < 2; ++a2)
+#pragma unroll
+        for (int b2 = 0; b2 < 2; ++b2) t[a2][b2] = Sib[a2][0] * Si[0][b2] + Sib[a2][1] * Si[1][b2];
+#pragma unroll
+      for (int a2 = 0; a2 < 2; ++a2)
+#pragma unroll
+        for (int b2 = 0; b2 < 2; ++b2) Sb[a2][b2] = -(Si[a2][0] * t[0][b2] + Si[a2][1] * t[1][b2]);
+      const double off = 0.5 * (Sb[0][1] + Sb[1][0]);
+      Cpb[0][0] += Sb[0][0];
+      Cpb[1][1] += Sb[1][1];
+      Cpb[0][1] += off;
+      Cpb[1][0] += off;
+    }
+    // W = Cp[:, 0:2] of the symmetric Cp
+#pragma unroll
+    for (int q = 0; q < 5; ++q)
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+        Cpb[q][j] += 0.5 * Wb[q][j];
+        Cpb[j][q] += 0.5 * Wb[q][j];
+      }
+    xpb[0] -= rb[0];
+    xpb[1] -= rb[1];
+
+    // ---- covariance transport Cp = jb Cm jb^T
+    double jbb[5][5];
+    {
+      double T[5][5];  // Cpb jb
+#pragma unroll
+      for (int q = 0; q < 5; ++q)
+#pragma unroll
+        for (int c = 0; c < 5; ++c) {
+          double s = 0.0;
+#pragma unroll
+          for (int m = 0; m < 5; ++m) s += Cpb[q][m] * jb[m][c];
+          T[q][c] = s;
+        }
+#pragma unroll
+      for (int q = 0; q < 5; ++q)
+#pragma unroll
+        for (int c = 0; c < 5; ++c) {
+          double s = 0.0, w2 = 0.0;
+#pragma unroll
+          for (int m = 0; m < 5; ++m) {
+            s += jb[m][q] * T[m][c];  // jb^T Cpb jb
+            w2 += T[q][m] * Cm[m][c];  // Cpb jb Cm
+          }
+          Cmb[q][c] = s;
+          jbb[q][c] = 2.0 * w2;
+        }
+    }
+    // ---- bound Jacobian rows
+    double l20b = 0, l21b = 0, l30b = 0, l31b = 0, l32b = 0, svb[5], nvb[3] = {0, 0, 0};
+    double u0b[3] = {0, 0, 0}, u1b[3] = {0, 0, 0};
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      posb[k] = 0.0;
+      db[k] = 0.0;
+      k4b[k] = 0.0;
+      cb[k] = 0.0;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) Gb[k][c] = 0.0;
+    }
+#pragma unroll
+    for (int c = 0; c < 5; ++c) {
+      double jpPb[3], jpDb[3];
+#pragma unroll
+      for (int q = 0; q < 3; ++q) {
+        jpPb[q] = jbb[0][c] * u0[q] + jbb[1][c] * u1[q];
+        u0b[q] += jbb[0][c] * jpP[q][c];
+        u1b[q] += jbb[1][c] * jpP[q][c];
+      }
+      jpDb[0] = jbb[2][c] * l20 + jbb[3][c] * l30;
+      jpDb[1] = jbb[2][c] * l21 + jbb[3][c] * l31;
+      jpDb[2] = jbb[3][c] * l32;
+      l20b += jbb[2][c] * jpD[0][c];
+      l21b += jbb[2][c] * jpD[1][c];
+      l30b += jbb[3][c] * jpD[0][c];
+      l31b += jbb[3][c] * jpD[1][c];
+      l32b += jbb[3][c] * jpD[2][c];
+      double s = 0.0;
+#pragma unroll
+      for (int q = 0; q < 3; ++q) {
+        Ab[q][c] = jpPb[q];
+        db[q] -= sv[c] * jpPb[q];
+        s -= d[q] * jpPb[q];
+        if (c >= 2) Gb[q][c - 2] = jpDb[q];
+        k4b[q] -= sv[c] * jpDb[q];
+        s -= k4[q] * jpDb[q];
+      }
+      svb[c] = s;
+    }
+    // sv[c] = nv . A[:,c], nv = nr / dp, dp = nr . d
+#pragma unroll
+    for (int c = 0; c < 5; ++c)
+#pragma unroll
+      for (int q = 0; q < 3; ++q) {
+        nvb[q] += svb[c] * A[q][c];
+        Ab[q][c] += svb[c] * nv[q];
+      }
+    double nrb[3];
+    const double dpb = -(nvb[0] * nr[0] + nvb[1] * nr[1] + nvb[2] * nr[2]) / (dp * dp);
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      nrb[q] = nvb[q] / dp + dpb * d[q];
+      db[q] += dpb * nr[q];
+    }
+    // l's as functions of d (rho^2 = d0^2 + d1^2)
+    {
+      const double r2 = rho * rho, r3 = r2 * rho, r4 = r2 * r2;
+      const double d0 = d[0], d1 = d[1], d2 = d[2];
+      db[0] += l20b * (2.0 * d1 * d0 / r4) + l21b * (1.0 / r2 - 2.0 * d0 * d0 / r4) +
+               l30b * (d2 / rho - d0 * d0 * d2 / r3) + l31b * (-d0 * d1 * d2 / r3) +
+               l32b * (-d0 / rho);
+      db[1] += l20b * (-1.0 / r2 + 2.0 * d1 * d1 / r4) + l21b * (-2.0 * d0 * d1 / r4) +
+               l30b * (-d0 * d1 * d2 / r3) + l31b * (d2 / rho - d1 * d1 * d2 / r3) +
+               l32b * (-d1 / rho);
+      db[2] += l30b * (d0 / rho) + l31b * (d1 / rho);
+      // parameters: phi = atan2(d1, d0), theta = atan2(rho, d2)
+      const double rr = r2 + d2 * d2;
+      db[0] += xpb[2] * (-d1 / r2) + xpb[3] * (d2 / rr) * (d0 / rho);
+      db[1] += xpb[2] * (d0 / r2) + xpb[3] * (d2 / rr) * (d1 / rho);
+      db[2] += xpb[3] * (-rho / rr);
+    }
+    // loc = (u0, u1) . (pos - c)
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      u0b[q] += xpb[0] * (pos[q] - cc[q]);
+      u1b[q] += xpb[1] * (pos[q] - cc[q]);
+      posb[q] += xpb[0] * u0[q] + xpb[1] * u1[q];
+      cb[q] -= xpb[0] * u0[q] + xpb[1] * u1[q];
+    }
+    qb += xpb[4];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      Rb[q][0] = u0b[q];
+      Rb[q][1] = u1b[q];
+      Rb[q][2] = nrb[q];
+    }
+  }
+};
+
+// Backward of the free state made from the filtered parameters on the plane
+// (updateFromBound): pos = c + u0 xf0 + u1 xf1, dir = dir(phi, theta),
+// q/p = q |xf4|.  Accumulates into xfb, cb, Rb.
+KFD_INLINE void freeFromBoundVJP(const PlaneFwd &P, double q, const double posb[3],
+                                 const double db[3], double qopb, double xfb[5], double cb[3],
+                                 double Rb[3][3]) {
+  const double ph = P.xf[2], th = P.xf[3];
+  const double cp = cos(ph), sp = sin(ph), ct = cos(th), st = sin(th);
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    xfb[0] += P.u0[k] * posb[k];
+    xfb[1] += P.u1[k] * posb[k];
+    Rb[k][0] += P.xf[0] * posb[k];
+    Rb[k][1] += P.xf[1] * posb[k];
+    cb[k] += posb[k];
+  }
+  xfb[2] += db[0] * (-sp * st) + db[1] * (cp * st);
+  xfb[3] += db[0] * (cp * ct) + db[1] * (sp * ct) + db[2] * (-st);
+  xfb[4] += qopb * q * ((P.xf[4] < 0.0) ? -1.0 : 1.0);
+}
+
+// Backward of the Jacobian re-initialisation on a plane (PlaneSurface.ipp:
+// 122-156): A[:,0] = u0, A[:,1] = u1, G[0][0] = -d1, G[0][1] = d2 d0 / rho,
+// G[1][0] = d0, G[1][1] = d2 d1 / rho, G[2][1] = -rho (d: direction at the
+// plane before the update).
+KFD_INLINE void jacInitVJP(const double d[3], const double Ab[3][5], const double Gb[3][3],
+                           double db[3], double Rb[3][3]) {
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    Rb[k][0] += Ab[k][0];
+    Rb[k][1] += Ab[k][1];
+  }
+  const double rho = sqrt(d[0] * d[0] + d[1] * d[1]), r3 = rho * rho * rho;
+  const double d0 = d[0], d1 = d[1], d2 = d[2];
+  db[0] += Gb[0][1] * (d2 / rho - d2 * d0 * d0 / r3) + Gb[1][0] +
+           Gb[1][1] * (-d2 * d1 * d0 / r3) + Gb[2][1] * (-d0 / rho);
+  db[1] += -Gb[0][0] + Gb[0][1] * (-d2 * d0 * d1 / r3) +
+           Gb[1][1] * (d2 / rho - d2 * d1 * d1 / r3) + Gb[2][1] * (-d1 / rho);
+  db[2] += Gb[0][1] * (d0 / rho) + Gb[1][1] * (d1 / rho);
+}
+
+// ---------------------------------------------------------------------------
+// The reverse sweep of one track.
+struct TrackAdjoint {
+  const FitArgs &a;
+  const AdjArgs &g;
+  const int64_t i;
+  double *stk;  // this wave's step stack
+  uint32_t lane;
+  TrackFit<double, true> tf;  // recomputes segments from the checkpoints
+
+  KFD_INLINE TrackAdjoint(const FitArgs &args, const AdjArgs &adj, int64_t idx, double *ckpt,
+                          double *stack, uint32_t ln)
+      : a(args), g(adj), i(idx), stk(stack), lane(ln),
+        tf(args, idx, nullptr, nullptr, nullptr, ckpt, ln) {}
+
+  KFD_INLINE double *sr(int j, int f) const {
+    return stk + (uint32_t)(((uint32_t)j * (uint32_t)kSR_N + (uint32_t)f) * 64u + lane);
+  }
+  KFD_INLINE double *pg(int q, int f) const {
+    return g.plane_adj + ((size_t)q * kPG_N + f) * (size_t)a.n + i;
+  }
+  KFD_INLINE void addPlane(int plane, const double cb[3], const double Rb[3][3]) const {
+    const int q = plane - tf.seqFirst;
+    if (q < 0 || q >= g.max_seq) return;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) *pg(q, kPG_C + k) += cb[k];
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) *pg(q, kPG_R + 3 * r + c) += Rb[r][c];
+  }
+  KFD_INLINE int64_t hitOn(int plane) const {
+    const int32_t id = a.planes[plane].geo_id;
+    for (int64_t h = tf.h0; h < tf.h1; ++h)
+      if (a.hit_geo_id[h] == id) return h;
+    return -1;
+  }
+
+  KFD_INLINE void run() {
+    tf.loadTrack();
+    const int nmeas = a.out_nmeas[i];
+    const int slots = (tf.size < g.max_seq) ? tf.size : g.max_seq;
+    for (int q = 0; q < slots; ++q)
+      for (int f = 0; f < kPG_N; ++f) *pg(q, f) = 0.0;
+    uint32_t st = 0;
+    double sb = 0.0;
+    if (nmeas <= 0) {
+      g.scale_adj[i] = 0.0;
+      if (g.status) g.status[i] = KF_ADJ_SKIPPED;
+      return;
+    }
+    // adjoint of the state after the update at the surface being processed
+    double pb[3] = {0, 0, 0}, db[3] = {0, 0, 0}, qb = 0.0;
+    double Ab[3][5], Gb[3][3], Cb[5][5];
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+#pragma unroll
+      for (int c = 0; c < 5; ++c) Ab[r][c] = 0.0;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) Gb[r][c] = 0.0;
+    }
+#pragma unroll
+    for (int r = 0; r < 5; ++r)
+#pragma unroll
+      for (int c = 0; c < 5; ++c) Cb[r][c] = 0.0;
+
+    // the stepper charge is set once from the start parameters and never
+    // updated (the checkpoints do not carry it)
+    const double charge = (a.start_params[(size_t)4 * a.n + i] > 0.0) ? 1.0 : -1.0;
+    for (int k = nmeas - 1; k >= 0 && st == 0; --k) {
+      // ---- recompute segment k from the checkpoint of surface k - 1
+      tf.result = true;
+      tf.finished = tf.smoothed = false;
+      tf.navBreak = false;
+      tf.stepFailed = false;
+      tf.targetReached = false;
+      tf.cur = -1;
+      if (k == 0) {
+        double pv[6];
+#pragma unroll
+        for (int m = 0; m < 6; ++m) pv[m] = a.start_params[(size_t)m * a.n + i];
+#pragma unroll
+        for (int m = 0; m < 21; ++m) tf.cov[m] = a.start_cov[(size_t)m * a.n + i];
+        tf.measured = 0;
+        tf.next = 0;
+        tf.start(pv);
+        tf.navStatus();
+        if (tf.cur >= 0 && hitOn(tf.cur) >= 0) {  // measured without a step
+          st |= KF_ADJ_LOST;
+          break;
+        }
+      } else {
+        tf.loadCheckpoint(k - 1);
+        tf.q = charge;
+        tf.measured = k;
+      }
+      tf.pathLimit();
+      tf.navTarget();
+      const double qop = tf.q / tf.p;
+      int nst = 0;
+      int plane = -1;
+      for (;;) {
+        if (nst >= g.max_seg_steps) {
+          st |= KF_ADJ_OVERFLOW;
+          break;
+        }
+#pragma unroll
+        for (int m = 0; m < 3; ++m) {
+          *sr(nst, kSR_POS + m) = tf.pos[m];
+          *sr(nst, kSR_DIR + m) = tf.dir[m];
+        }
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+#pragma unroll
+          for (int c = 0; c < 3; ++c) *sr(nst, kSR_G + 3 * r + c) = tf.G[r][c];
+        const int target = (tf.next < tf.size) ? tf.seqFirst + tf.next : -1;
+        if (!tf.step()) {
+          st |= KF_ADJ_LOST;
+          break;
+        }
+        const double h = tf.ss.get();
+        *sr(nst, kSR_H) = h;
+        *sr(nst, kSR_TGT) = (target >= 0 && h == tf.ss.v[kActor]) ? (double)(target + 1) : 0.0;
+        ++nst;
+        tf.navStatus();
+        if (tf.cur >= 0 && hitOn(tf.cur) >= 0) {
+          plane = tf.cur;
+          break;
+        }
+        if (tf.pathLimit()) {
+          st |= KF_ADJ_LOST;
+          break;
+        }
+        tf.navTarget();
+        if (tf.navBreak) {
+          st |= KF_ADJ_LOST;
+          break;
+        }
+      }
+      if (st) break;
+      // ---- bound state, chi2, update at the measured plane
+      const DevPlane &pl = a.planes[plane];
+      const int64_t hit = hitOn(plane);
+      double dEnd[3], posEnd[3], k4[3], AEnd[3][5], GEnd[3][3];
+#pragma unroll
+      for (int m = 0; m < 3; ++m) {
+        dEnd[m] = tf.dir[m];
+        posEnd[m] = tf.pos[m];
+        k4[m] = tf.k4d[m];
+      }
+#pragma unroll
+      for (int r = 0; r < 3; ++r) {
+#pragma unroll
+        for (int c = 0; c < 5; ++c) AEnd[r][c] = tf.A[r][c];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) GEnd[r][c] = tf.G[r][c];
+      }
+      PlaneFwd P;
+      P.run(pl, dEnd, posEnd, AEnd, GEnd, k4, qop, tf.cov, a.hit_meas[hit],
+            a.hit_meas[a.n_hits + hit], a.hit_cov[hit], a.hit_cov[a.n_hits + hit],
+            a.hit_cov[2 * a.n_hits + hit]);
+      // ---- backward at the plane
+      double cb[3] = {0, 0, 0}, Rb[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+      double dPre[3] = {0, 0, 0};  // adjoint of the direction at the plane (before the update)
+      jacInitVJP(dEnd, Ab, Gb, dPre, Rb);
+      double xfb[5] = {0, 0, 0, 0, 0};
+      freeFromBoundVJP(P, tf.q, pb, db, qb, xfb, cb, Rb);
+      double posb[3], dbb[3], Abb[3][5], Gbb[3][3], k4b[3], qbb = 0.0, Cmb[5][5], cb2[3],
+          Rb2[3][3];
+      P.vjp(dEnd, posEnd, AEnd, k4, xfb, Cb, posb, dbb, Abb, Gbb, k4b, qbb, Cmb, cb2, Rb2);
+#pragma unroll
+      for (int m = 0; m < 3; ++m) {
+        cb[m] += cb2[m];
+        dbb[m] += dPre[m];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) Rb[m][c] += Rb2[m][c];
+      }
+      addPlane(plane, cb, Rb);
+      // ---- RK steps in reverse
+      for (int j = nst - 1; j >= 0; --j) {
+        double pin[3], din[3], Gin[3][3];
+#pragma unroll
+        for (int m = 0; m < 3; ++m) {
+          pin[m] = *sr(j, kSR_POS + m);
+          din[m] = *sr(j, kSR_DIR + m);
+        }
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+#pragma unroll
+          for (int c = 0; c < 3; ++c) Gin[r][c] = *sr(j, kSR_G + 3 * r + c);
+        const double h = *sr(j, kSR_H);
+        const int tgt = (int)*sr(j, kSR_TGT) - 1;
+        StepFwd F;
+        F.run(a.field, pin, din, h, qop);
+        double hb = 0.0;
+        stepVJP(a.field, F, pin, din, h, qop, Gin, posb, dbb, Gbb, Abb, k4b, hb, qbb, sb);
+#pragma unroll
+        for (int m = 0; m < 3; ++m) k4b[m] = 0.0;
+        if (tgt >= 0) {
+          // h = n . (c - pos) / (n . d) towards plane tgt
+          const DevPlane &tp = a.planes[tgt];
+          const double n[3] = {tp.r[2], tp.r[5], tp.r[8]};
+          const double den = n[0] * din[0] + n[1] * din[1] + n[2] * din[2];
+          double tcb[3], tRb[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+#pragma unroll
+          for (int m = 0; m < 3; ++m) {
+            const double g1 = hb * n[m] / den;
+            posb[m] -= g1;
+            dbb[m] -= g1 * h;
+            tcb[m] = g1;
+            tRb[m][2] = hb * ((tp.c[m] - pin[m]) - h * din[m]) / den;
+          }
+          addPlane(tgt, tcb, tRb);
+        }
+      }
+      // ---- adjoint of the state after the update at the previous surface
+#pragma unroll
+      for (int m = 0; m < 3; ++m) {
+        pb[m] = posb[m];
+        db[m] = dbb[m];
+      }
+      qb = qbb;
+#pragma unroll
+      for (int r = 0; r < 3; ++r) {
+#pragma unroll
+        for (int c = 0; c < 5; ++c) Ab[r][c] = Abb[r][c];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) Gb[r][c] = Gbb[r][c];
+      }
+#pragma unroll
+      for (int r = 0; r < 5; ++r)
+#pragma unroll
+        for (int c = 0; c < 5; ++c) Cb[r][c] = Cmb[r][c];
+    }
+    if (st) {
+      for (int q = 0; q < slots; ++q)
+        for (int f = 0; f < kPG_N; ++f) *pg(q, f) = 0.0;
+      sb = 0.0;
+    }
+    g.scale_adj[i] = sb;
+    if (g.status) g.status[i] = st;
+  }
+};
+
+}  // namespace kfd

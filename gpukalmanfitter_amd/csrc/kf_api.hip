@@ -14,6 +14,7 @@
 #include <vector>
 
 #include "../../include/kf_abi.h"
+#include "kf_adjoint.hpp"
 #include "kf_kernels.hpp"
 
 struct kf_ctx {
@@ -29,6 +30,9 @@ struct kf_ctx {
   size_t store_bytes = 0;
   void *carry = nullptr;    // forward -> backward kernel hand-over
   size_t carry_bytes = 0;
+  void *adj = nullptr;      // chi2-adjoint buffers (step stack, plane adjoints, ...)
+  size_t adj_bytes = 0;
+  bool has_material = false;  // a plane of the table carries material
   hipEvent_t ev0 = nullptr, evm = nullptr, ev1 = nullptr;
   bool timed = false;
 };
@@ -146,6 +150,7 @@ int kf_ctx_destroy(kf_ctx *ctx) {
   if (ctx->grid) (void)hipFree(ctx->grid);
   if (ctx->store) (void)hipFree(ctx->store);
   if (ctx->carry) (void)hipFree(ctx->carry);
+  if (ctx->adj) (void)hipFree(ctx->adj);
   if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
   if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
   if (ctx->evm) (void)hipEventDestroy(ctx->evm);
@@ -188,6 +193,8 @@ int kf_set_geometry(kf_ctx *ctx, const kf_plane *planes, int32_t n_planes) {
   KF_HIP(ctx, hipMemcpy(ctx->planes, dp.data(), sizeof(kfd::DevPlane) * n_planes,
                         hipMemcpyHostToDevice));
   ctx->n_planes = n_planes;
+  ctx->has_material = false;
+  for (const kfd::DevPlane &d : dp) ctx->has_material = ctx->has_material || d.has_material;
   return KF_OK;
 }
 
@@ -245,29 +252,27 @@ int kf_set_field_scale(kf_ctx *ctx, double s) {
   return KF_OK;
 }
 
-int kf_fit(kf_ctx *ctx, const kf_tracks *tr, const kf_options *opt, const kf_results *res,
-           void *stream) {
-  if (!ctx) return fail(nullptr, KF_E_INVALID, "kf_fit: ctx is NULL");
-  if (!tr || !opt || !res) return fail(ctx, KF_E_INVALID, "kf_fit: NULL tracks/options/results");
-  if (!ctx->planes) return fail(ctx, KF_E_STATE, "kf_fit: no geometry (kf_set_geometry)");
-  if (!ctx->field_set) return fail(ctx, KF_E_STATE, "kf_fit: no field (kf_set_field_*)");
-  if (tr->n_tracks < 0 || tr->n_hits < 0) return fail(ctx, KF_E_INVALID, "kf_fit: negative size");
-  if (tr->n_tracks == 0) return KF_OK;
-  if (!tr->start_params || !tr->start_cov || !tr->perigee || !tr->seq_first || !tr->seq_count ||
-      !tr->hit_offset)
-    return fail(ctx, KF_E_INVALID, "kf_fit: NULL per-track input array");
+}  // extern "C"
+
+namespace {
+
+// Checks the batch and options and fills the kernel arguments shared by the
+// fit and the adjoint (inputs, options, physics constants).
+int setupArgs(kf_ctx *ctx, const kf_tracks *tr, const kf_options *opt, const char *who,
+              kfd::FitArgs &a) {
+  if (!ctx->planes) return fail(ctx, KF_E_STATE, "%s: no geometry (kf_set_geometry)", who);
+  if (!ctx->field_set) return fail(ctx, KF_E_STATE, "%s: no field (kf_set_field_*)", who);
+  if (tr->n_tracks < 0 || tr->n_hits < 0) return fail(ctx, KF_E_INVALID, "%s: negative size", who);
+  if (tr->n_tracks > 0 && (!tr->start_params || !tr->start_cov || !tr->perigee ||
+                           !tr->seq_first || !tr->seq_count || !tr->hit_offset))
+    return fail(ctx, KF_E_INVALID, "%s: NULL per-track input array", who);
   if (tr->n_hits > 0 && (!tr->hit_geo_id || !tr->hit_meas || !tr->hit_cov))
-    return fail(ctx, KF_E_INVALID, "kf_fit: NULL hit array");
-  if (!res->params || !res->cov || !res->status)
-    return fail(ctx, KF_E_INVALID, "kf_fit: results need params, cov and status");
+    return fail(ctx, KF_E_INVALID, "%s: NULL hit array", who);
   if (opt->precision != KF_PRECISION_FP64 && opt->precision != KF_PRECISION_MIXED)
-    return fail(ctx, KF_E_INVALID, "kf_fit: unknown precision %d", opt->precision);
+    return fail(ctx, KF_E_INVALID, "%s: unknown precision %d", who, opt->precision);
   if (opt->max_steps < 0 || opt->max_rk_trials < 0 || opt->max_states < 0 || !(opt->mass > 0.0) ||
       !(opt->overstep_limit >= 0.0))
-    return fail(ctx, KF_E_INVALID, "kf_fit: invalid options");
-
-  const int32_t max_states = opt->max_states > 0 ? opt->max_states : ctx->n_planes;
-  kfd::FitArgs a;
+    return fail(ctx, KF_E_INVALID, "%s: invalid options", who);
   std::memset(&a, 0, sizeof a);
   a.n = tr->n_tracks;
   a.n_hits = tr->n_hits;
@@ -288,7 +293,7 @@ int kf_fit(kf_ctx *ctx, const kf_tracks *tr, const kf_options *opt, const kf_res
   a.eloss = opt->energy_loss;
   a.max_steps = opt->max_steps;
   a.max_rk_trials = (uint32_t)opt->max_rk_trials;
-  a.max_states = max_states;
+  a.max_states = opt->max_states > 0 ? opt->max_states : ctx->n_planes;
   a.mass = opt->mass;
   a.tolerance = opt->tolerance;
   a.step_size_cutoff = opt->step_size_cutoff;
@@ -296,43 +301,134 @@ int kf_fit(kf_ctx *ctx, const kf_tracks *tr, const kf_options *opt, const kf_res
   a.target_tolerance = opt->target_tolerance;
   a.overstep_limit = opt->overstep_limit;
   physicsConstants(a);
+  return KF_OK;
+}
+
+// Per-state stores (layout in kf_kernels.hpp) and the forward -> backward
+// hand-over, carved from the context's buffers.
+int setupStores(kf_ctx *ctx, int precision, bool track_states, bool checkpoints, hipStream_t s,
+                kfd::FitArgs &a) {
+  const size_t slots = ((size_t)a.n + 63) / 64 * 64 * (size_t)a.max_states;
+  const size_t s_size = (precision == KF_PRECISION_FP64) ? sizeof(double) : sizeof(float);
+  const size_t d_bytes = slots * KF_SD_N * sizeof(double);
+  const size_t s_bytes = (slots * KF_SS_N * s_size + 255) / 256 * 256;
+  const size_t f_bytes = track_states ? slots * KF_TS_STRIDE * sizeof(double) : 0;
+  const size_t c_bytes = checkpoints ? slots * KF_CK_N * sizeof(double) : 0;
+  int rc = grow(ctx, &ctx->store, &ctx->store_bytes, d_bytes + s_bytes + f_bytes + c_bytes, s,
+                "state store");
+  if (rc != KF_OK) return rc;
+  char *base = (char *)ctx->store;
+  a.tss = base;
+  a.tsd = (double *)(base + s_bytes);
+  a.tsf = track_states ? (double *)(base + s_bytes + d_bytes) : nullptr;
+  a.tsc = checkpoints ? (double *)(base + s_bytes + d_bytes + f_bytes) : nullptr;
+  rc = grow(ctx, &ctx->carry, &ctx->carry_bytes,
+            (size_t)a.n * (6 * sizeof(int32_t) + 30 * sizeof(double)), s, "hand-over buffer");
+  if (rc != KF_OK) return rc;
+  a.carry_d = (double *)ctx->carry;
+  a.carry_i = (int32_t *)((char *)ctx->carry + (size_t)a.n * 30 * sizeof(double));
+  return KF_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int kf_fit(kf_ctx *ctx, const kf_tracks *tr, const kf_options *opt, const kf_results *res,
+           void *stream) {
+  if (!ctx) return fail(nullptr, KF_E_INVALID, "kf_fit: ctx is NULL");
+  if (!tr || !opt || !res) return fail(ctx, KF_E_INVALID, "kf_fit: NULL tracks/options/results");
+  kfd::FitArgs a;
+  int rc = setupArgs(ctx, tr, opt, "kf_fit", a);
+  if (rc != KF_OK) return rc;
+  if (tr->n_tracks == 0) return KF_OK;
+  if (!res->params || !res->cov || !res->status)
+    return fail(ctx, KF_E_INVALID, "kf_fit: results need params, cov and status");
+  if (res->track_states && !res->n_measured)
+    return fail(ctx, KF_E_INVALID, "kf_fit: track_states needs results.n_measured");
   a.out_params = res->params;
   a.out_cov = res->cov;
   a.out_chi2 = res->chi2;
   a.out_ndf = res->ndf;
   a.out_nmeas = res->n_measured;
   a.out_status = res->status;
-
   KF_HIP(ctx, hipSetDevice(ctx->device));
-  if (res->track_states && !res->n_measured)
-    return fail(ctx, KF_E_INVALID, "kf_fit: track_states needs results.n_measured");
   hipStream_t s = (hipStream_t)stream;
-  {
-    // per-state stores, chunked by 64 tracks (layout in kf_kernels.hpp)
-    const size_t slots = ((size_t)tr->n_tracks + 63) / 64 * 64 * (size_t)max_states;
-    const size_t s_size = (opt->precision == KF_PRECISION_FP64) ? sizeof(double) : sizeof(float);
-    const size_t d_bytes = slots * KF_SD_N * sizeof(double);
-    const size_t s_bytes = (slots * KF_SS_N * s_size + 255) / 256 * 256;
-    const size_t f_bytes = res->track_states ? slots * KF_TS_STRIDE * sizeof(double) : 0;
-    int rc = grow(ctx, &ctx->store, &ctx->store_bytes, d_bytes + s_bytes + f_bytes, s,
-                  "state store");
-    if (rc != KF_OK) return rc;
-    char *base = (char *)ctx->store;
-    a.tss = base;
-    a.tsd = (double *)(base + s_bytes);
-    a.tsf = res->track_states ? (double *)(base + s_bytes + d_bytes) : nullptr;
-  }
-  {
-    int rc = grow(ctx, &ctx->carry, &ctx->carry_bytes,
-                  (size_t)tr->n_tracks * (6 * sizeof(int32_t) + 30 * sizeof(double)), s,
-                  "hand-over buffer");
-    if (rc != KF_OK) return rc;
-    a.carry_d = (double *)ctx->carry;
-    a.carry_i = (int32_t *)((char *)ctx->carry + (size_t)tr->n_tracks * 30 * sizeof(double));
-  }
+  rc = setupStores(ctx, opt->precision, res->track_states != nullptr, false, s, a);
+  if (rc != KF_OK) return rc;
   KF_HIP(ctx, hipEventRecord(ctx->ev0, s));
   hipError_t e = kfd_launch_fit(&a, opt->precision, s, res->track_states, ctx->evm);
   if (e != hipSuccess) return fail(ctx, KF_E_HIP, "kf_fit: launch failed: %s", hipGetErrorString(e));
+  KF_HIP(ctx, hipEventRecord(ctx->ev1, s));
+  ctx->timed = true;
+  return KF_OK;
+}
+
+int kf_chi2_gradient(kf_ctx *ctx, const kf_tracks *tr, const kf_options *opt, double *grad_planes,
+                     double *grad_field_scale, double *chi2_sum, double *grad_raw,
+                     uint32_t *track_flags, void *stream) {
+  if (!ctx) return fail(nullptr, KF_E_INVALID, "kf_chi2_gradient: ctx is NULL");
+  if (!tr || !opt || !grad_planes || !grad_field_scale)
+    return fail(ctx, KF_E_INVALID, "kf_chi2_gradient: NULL tracks/options/outputs");
+  kfd::FitArgs a;
+  int rc = setupArgs(ctx, tr, opt, "kf_chi2_gradient", a);
+  if (rc != KF_OK) return rc;
+  if ((opt->multiple_scattering || opt->energy_loss) && ctx->has_material)
+    return fail(ctx, KF_E_INVALID,
+                "kf_chi2_gradient: material effects are not differentiated; disable "
+                "multiple_scattering and energy_loss or use planes without material");
+  a.smoothing = 0;  // the objective is the forward filter's chi2
+  KF_HIP(ctx, hipSetDevice(ctx->device));
+  hipStream_t s = (hipStream_t)stream;
+  rc = setupStores(ctx, KF_PRECISION_FP64, false, true, s, a);
+  if (rc != KF_OK) return rc;
+  const int32_t max_seg = KF_ADJ_MAX_SEGMENT_STEPS;
+  const size_t n = (size_t)tr->n_tracks;
+  const size_t chunks = (n + 63) / 64;
+  const size_t nplanes = (size_t)ctx->n_planes;
+  // forward outputs (params, cov, chi2, ndf, n_measured, status), step
+  // stack, per-track plane adjoints, field-scale adjoints, flags, sums
+  const size_t b_out = n * (6 + 21 + 1) * sizeof(double) + n * 3 * sizeof(int32_t);
+  const size_t b_stack = chunks * 64 * (size_t)max_seg * kfd::kSR_N * sizeof(double);
+  const size_t b_pg = nplanes * kfd::kPG_N * n * sizeof(double);
+  const size_t b_scale = n * sizeof(double);
+  const size_t b_flags = n * sizeof(uint32_t);
+  const size_t b_sums = (nplanes + 2) * kfd::kPG_N * sizeof(double);
+  auto al = [](size_t x) { return (x + 255) / 256 * 256; };
+  rc = grow(ctx, &ctx->adj, &ctx->adj_bytes,
+            al(b_out) + al(b_stack) + al(b_pg) + al(b_scale) + al(b_flags) + al(b_sums), s,
+            "adjoint buffers");
+  if (rc != KF_OK) return rc;
+  char *p = (char *)ctx->adj;
+  a.out_params = (double *)p;
+  a.out_cov = a.out_params + 6 * n;
+  a.out_chi2 = a.out_cov + 21 * n;
+  a.out_ndf = (int32_t *)(a.out_chi2 + n);
+  a.out_nmeas = a.out_ndf + n;
+  a.out_status = (uint32_t *)(a.out_nmeas + n);
+  p += al(b_out);
+  kfd::AdjArgs g;
+  g.stack = (double *)p;
+  p += al(b_stack);
+  g.max_seg_steps = max_seg;
+  g.max_seq = (int32_t)nplanes;
+  g.plane_adj = (double *)p;
+  p += al(b_pg);
+  g.scale_adj = (double *)p;
+  p += al(b_scale);
+  g.status = track_flags ? track_flags : (uint32_t *)p;
+  p += al(b_flags);
+  double *sums = (double *)p;
+  KF_HIP(ctx, hipEventRecord(ctx->ev0, s));
+  hipError_t e = (n > 0) ? kfd_launch_fit(&a, KF_PRECISION_FP64, s, nullptr, ctx->evm) : hipSuccess;
+  if (e == hipSuccess && n == 0) e = hipEventRecord(ctx->evm, s);
+  if (e == hipSuccess)
+    e = kfd_launch_adjoint(&a, &g, sums, grad_planes, grad_field_scale, chi2_sum, s);
+  if (e == hipSuccess && grad_raw)
+    e = hipMemcpyAsync(grad_raw, sums, nplanes * kfd::kPG_N * sizeof(double),
+                       hipMemcpyDeviceToDevice, s);
+  if (e != hipSuccess)
+    return fail(ctx, KF_E_HIP, "kf_chi2_gradient: launch failed: %s", hipGetErrorString(e));
   KF_HIP(ctx, hipEventRecord(ctx->ev1, s));
   ctx->timed = true;
   return KF_OK;

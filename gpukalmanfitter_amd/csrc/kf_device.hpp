@@ -109,6 +109,76 @@ KFD_INLINE void fieldAt(const DevField &f, double x, double y, double z, double 
   }
 }
 
+// Field and its spatial derivative dB[k][axis] = dB_k / dx_axis (the chi2
+// adjoint).  Same interpolation as fieldAt; an axis clamped to the box edge
+// has zero derivative.
+KFD_INLINE void fieldAtGrad(const DevField &f, const double x[3], double B[3], double dB[3][3]) {
+  if (!f.grid) {
+    fieldAt(f, x[0], x[1], x[2], B);
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+#pragma unroll
+      for (int d = 0; d < 3; ++d) dB[k][d] = 0.0;
+    return;
+  }
+  int idx[3];
+  double fr[3], dfr[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    double v = x[d];
+    bool clamped = false;
+    if (v < f.lo[d]) {
+      v = f.lo[d];
+      clamped = true;
+    }
+    if (f.hi[d] < v) {
+      v = f.hi[d];
+      clamped = true;
+    }
+    int i = (int)floor((v - f.lo[d]) / f.w[d]);
+    i = i < 0 ? 0 : i;
+    i = i > f.n[d] - 2 ? f.n[d] - 2 : i;
+    idx[d] = i;
+    const double ll = f.lo[d] + (double)i * f.w[d];
+    const double ur = f.lo[d] + (double)(i + 1) * f.w[d];
+    fr[d] = (v - ll) / (ur - ll);
+    dfr[d] = clamped ? 0.0 : 1.0 / (ur - ll);
+  }
+  const double sc = f.scale;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    double val[2][2][2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+      for (int b = 0; b < 2; ++b)
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+          const size_t o = ((((size_t)(idx[0] + a)) * f.n[1] + (idx[1] + b)) * f.n[2] + (idx[2] + c)) * 3;
+          val[a][b][c] = (double)f.values[o + k];
+        }
+    const double wx[2] = {1.0 - fr[0], fr[0]}, wy[2] = {1.0 - fr[1], fr[1]},
+                 wz[2] = {1.0 - fr[2], fr[2]};
+    double s = 0.0, gx = 0.0, gy = 0.0, gz = 0.0;
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+      for (int b = 0; b < 2; ++b)
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+          const double v = val[a][b][c];
+          s += wx[a] * wy[b] * wz[c] * v;
+          gx += (a ? 1.0 : -1.0) * wy[b] * wz[c] * v;
+          gy += wx[a] * (b ? 1.0 : -1.0) * wz[c] * v;
+          gz += wx[a] * wy[b] * (c ? 1.0 : -1.0) * v;
+        }
+    B[k] = sc * s;
+    dB[k][0] = sc * gx * dfr[0];
+    dB[k][1] = sc * gy * dfr[1];
+    dB[k][2] = sc * gz * dfr[2];
+  }
+}
+
 // ------------------------------------------------------- small helpers ----
 template <typename T>
 KFD_INLINE void cross(const T a[3], const T b[3], T r[3]) {

@@ -17,6 +17,18 @@
 #define KF_SS_MD 21  // covariance (2,2), (3,3), (4,4) after the material update
 #define KF_SS_J 24   // Jacobian rows 0, 1, 2, 3, 5 x columns 0..4 (25)
 #define KF_SS_N 49
+// per-surface checkpoints of the chi2 adjoint (state after the update and
+// material at a measured surface; see FitArgs::tsc)
+#define KF_CK_POS 0    // 3
+#define KF_CK_DIR 3    // 3
+#define KF_CK_P 6
+#define KF_CK_T 7
+#define KF_CK_PATH 8
+#define KF_CK_SS 9     // ConstrainedStep values (4)
+#define KF_CK_COV 13   // 21
+#define KF_CK_J 34     // A (15), b, G (9)
+#define KF_CK_NEXT 59  // navigation index (as a double)
+#define KF_CK_N 60
 
 namespace kfd {
 
@@ -66,6 +78,9 @@ struct FitArgs {
   double *tsd;
   void *tss;
   double *tsf;
+  //  tsc  double, F = KF_CK_N: per-surface checkpoints of the chi2 adjoint
+  //       (NULL outside the adjoint).
+  double *tsc;
   // forward -> backward kernel hand-over, [field][n]
   int32_t *carry_i;  // 6 int fields
   double *carry_d;   // 30 double fields
@@ -76,5 +91,18 @@ struct FitArgs {
 // Launches the forward kernel, records `between` (if not NULL), launches the
 // smoother/back-propagation kernel, then the track-state export if
 // track_states_out is not NULL.
+// Step-stack depth per track of the chi2 adjoint: a segment between two
+// measured surfaces may take this many RK steps.
+#define KF_ADJ_MAX_SEGMENT_STEPS 48
+
+namespace kfd {
+struct AdjArgs;
+}
+// Launches the reverse sweep, the reduction and the conversion of the chi2
+// adjoint (kf_adjoint.hip); the forward filter with checkpoints must have run.
+extern "C" hipError_t kfd_launch_adjoint(const kfd::FitArgs *args, const kfd::AdjArgs *adj,
+                                         double *sums, double *grad_planes, double *grad_scale,
+                                         double *chi2, hipStream_t stream);
+
 extern "C" hipError_t kfd_launch_fit(const kfd::FitArgs *args, int precision, hipStream_t stream,
                                      double *track_states_out, hipEvent_t between);

@@ -54,6 +54,9 @@ def _lib():
         lib.kf_last_kernel_ms.restype = C.c_double
         lib.kf_last_phase_ms.argtypes = [P, P, P]
         lib.kf_last_phase_ms.restype = C.c_int
+        lib.kf_chi2_gradient.argtypes = [P, C.POINTER(abi.kf_tracks), C.POINTER(abi.kf_options),
+                                         P, P, P, P, P, P]
+        lib.kf_chi2_gradient.restype = C.c_int
         lib._kf_typed = True
     return lib
 
@@ -275,6 +278,46 @@ class KalmanFitter:
         s = stream if stream is not None else torch.cuda.current_stream(dev_batch.device)
         self._check(self._lib.kf_fit(self._ctx, C.byref(tracks), C.byref(opts), C.byref(results),
                                      C.c_void_p(s.cuda_stream)))
+
+    def set_geometry(self, planes):
+        """Replaces the plane table (same number of planes or not)."""
+        self.n_planes = len(planes)
+        self._check(self._lib.kf_set_geometry(self._ctx, planes, len(planes)))
+
+    def chi2_gradient(self, dev_batch, options=None, stream=None, synchronize=True):
+        """Gradient of the summed forward-filter chi2 of the batch with respect
+        to the alignment of every plane (local shifts du, dv, dw and rotations
+        a, b, c about the local axes, at zero) and to the field scale, by the
+        library's adjoint kernels (FP64).
+
+        Returns ``(grad_planes, grad_field_scale, chi2_sum, track_flags,
+        grad_raw)``: ``grad_planes`` is (n_planes, 6); ``track_flags`` holds
+        the KF_ADJ_* bits of tracks left out (0 = included); ``grad_raw`` is
+        (n_planes, 12), the derivatives with respect to each plane's centre (3)
+        and rotation matrix (9, row-major).  Device tensors when
+        ``synchronize`` is False, else numpy / floats.
+        """
+        torch = _torch()
+        opts = (options or KalmanFitterOptions()).to_c()
+        if opts.max_states == 0:
+            opts.max_states = dev_batch.max_states
+        dev = dev_batch.device
+        gp = torch.zeros((self.n_planes, 6), dtype=torch.float64, device=dev)
+        gs = torch.zeros(1, dtype=torch.float64, device=dev)
+        cs = torch.zeros(1, dtype=torch.float64, device=dev)
+        fl = torch.zeros(max(dev_batch.n_tracks, 1), dtype=torch.int32, device=dev)
+        raw = torch.zeros((self.n_planes, 12), dtype=torch.float64, device=dev)
+        tracks = dev_batch.c_tracks()
+        s = stream if stream is not None else torch.cuda.current_stream(dev)
+        self._check(self._lib.kf_chi2_gradient(self._ctx, C.byref(tracks), C.byref(opts),
+                                               gp.data_ptr(), gs.data_ptr(), cs.data_ptr(),
+                                               raw.data_ptr(), fl.data_ptr(),
+                                               C.c_void_p(s.cuda_stream)))
+        if not synchronize:
+            return gp, gs, cs, fl, raw
+        s.synchronize()
+        return (gp.cpu().numpy(), float(gs.item()), float(cs.item()),
+                fl[:dev_batch.n_tracks].cpu().numpy().view(np.uint32), raw.cpu().numpy())
 
     def last_kernel_ms(self):
         """Duration of the last fit's kernels (both launches), ms."""

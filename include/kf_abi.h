@@ -186,6 +186,32 @@ int kf_set_field_scale(kf_ctx *ctx, double s);
 int kf_fit(kf_ctx *ctx, const kf_tracks *tracks, const kf_options *opt,
            const kf_results *res, void *stream);
 
+/* Gradient of the summed chi2 of the forward filter (the fit's results.chi2
+ * summed over the batch) with respect to the alignment of every plane of the
+ * table and to the field scale, by a hand-written adjoint (reverse sweep over
+ * per-surface checkpoints), asynchronously on `stream`, always in FP64.
+ *   grad_planes       device [n_planes][6]: d/d(du, dv, dw, a, b, c) at zero,
+ *                     with the plane moved as centre' = centre + R (du, dv, dw)
+ *                     and R' = R Rx(a) Ry(b) Rz(c) (local shifts [mm] and
+ *                     rotations about the local U, V, normal axes [rad]);
+ *   grad_field_scale  device [1]: d/ds of the field B -> s B at the current s;
+ *   chi2_sum          device [1] or NULL: the objective;
+ *   grad_raw          device [n_planes][12] or NULL: dX/d(centre) (3) and
+ *                     dX/d(rot) (9, row-major) of every plane, for callers
+ *                     with their own alignment parameterisation;
+ *   track_flags       device [n_tracks] or NULL: 0, or KF_ADJ_* bits of a
+ *                     track left out of the gradient.
+ * Material effects are not differentiated: with multiple_scattering or
+ * energy_loss on and a plane with material the call fails.  No reference
+ * counterpart (this build's addition; the objective is defined in
+ * results.chi2). */
+#define KF_ADJ_SKIPPED 0x1u   /* no measured surface */
+#define KF_ADJ_OVERFLOW 0x2u  /* a segment took more RK steps than the adjoint stacks */
+#define KF_ADJ_LOST 0x4u      /* the recomputed segment did not reach its surface */
+int kf_chi2_gradient(kf_ctx *ctx, const kf_tracks *tracks, const kf_options *opt,
+                     double *grad_planes, double *grad_field_scale, double *chi2_sum,
+                     double *grad_raw, uint32_t *track_flags, void *stream);
+
 /* Duration of the last fit kernel on ctx in milliseconds (HIP events recorded
  * on the launch stream); < 0 if unknown. */
 double kf_last_kernel_ms(const kf_ctx *ctx);
