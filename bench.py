@@ -61,6 +61,8 @@ def parse():
     ap.add_argument("--cpu-sample", type=int, default=1048576,
                     help="tracks of the same workload timed on the CPU oracle")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-ragged", action="store_true",
+                    help="skip the ragged-batch leg (config 3, 2,000,000 tracks)")
     ap.add_argument("--no-adjoint", action="store_true",
                     help="skip the chi2-adjoint leg (config 4, 262,144 tracks)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -130,8 +132,8 @@ def roofline(prec, n, phase_ms):
     other = "backward" if k == "forward" else "forward"
     return {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
             "frac": achieved / peak, "traffic": TRAFFIC_BYTES_PER_LAUNCH.get((prec, k)),
-            "kernel": {"forward": "kf_forward_kernel", "backward": "kf_backward_kernel"}[k]
-                      + ("<float>" if mixed else "<double>"),
+            "kernel": "kf_fit_kernel<%s, %s>" % ("float" if mixed else "double",
+                                                  "true" if k == "forward" else "false"),
             "kernel_ms": ms[k], "flops_per_track": f, "fp32_flops_per_track": g,
             "compute": "vector ALU, not MFMA: peak = F / (F32/157.3 + F64/78.6 TFLOP/s), the "
                        "MI355X fp32/fp64 vector peaks weighted by this launch's flop mix",
@@ -174,6 +176,44 @@ def adjoint_leg(steps, warmup):
             "tracks_per_s": b.n_tracks * steps / el, "ms_per_step": el * 1e3 / steps,
             "kernel_ms": float(np.mean(kms)), "tracks_left_out": int((flags != 0).sum()),
             "dtype": "f64"}
+
+
+def ragged_leg(steps, warmup, precision="mixed"):
+    """The ragged batch: 2,000,000 tracks over 5-30 planes with 5 % holes in
+    the 101x101x201 grid field, fitted with the schedule the library's
+    autotuner picks on this batch (length sort, persistent work queue, graph
+    replay), beside the default one-thread-per-track schedule."""
+    import torch
+    from gpukalmanfitter_amd import configs
+    from gpukalmanfitter_amd import generate as G
+    from gpukalmanfitter_amd.fitter import DeviceBatch, KalmanFitter, KalmanFitterOptions
+    t0 = time.time()
+    b = G.generate(3)
+    print(f"[bench] generated {b.n_tracks} ragged tracks in {time.time() - t0:.1f}s",
+          file=sys.stderr, flush=True)
+    kf = KalmanFitter(b.planes, b.field)
+    dev = DeviceBatch(b)
+    opts = KalmanFitterOptions(precision=precision, **configs.WORKLOAD_FIT_OPTIONS)
+
+    def timed():
+        for _ in range(warmup):
+            kf.fit_batch(dev, opts)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(steps):
+            kf.fit_batch(dev, opts)
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t) / steps
+
+    default_s = timed()
+    best, tune_ms = kf.autotune(dev, opts)
+    tuned_s = timed()
+    kf.close()
+    return {"workload": "config3-ragged-grid: 2000000 tracks, U{5..30} planes at 25 mm pitch, "
+                        "5% holes, 101x101x201 FP32 grid field", "precision": precision,
+            "tracks_per_s": b.n_tracks / tuned_s, "ms_per_step": tuned_s * 1e3,
+            "schedule": best, "autotune_best_kernel_ms": tune_ms,
+            "default_schedule_tracks_per_s": b.n_tracks / default_s}
 
 
 def main():
@@ -240,6 +280,12 @@ def main():
 
     head = precisions[0]
     roof = roofline(head, n, phase_ms[head]) if args.config == 2 else None
+    ragged = None
+    if not args.no_ragged:
+        ragged = ragged_leg(max(args.steps // 2, 3), 1)
+        print(f"[bench] ragged: {ragged['tracks_per_s']:.4g} tracks/s autotuned "
+              f"({ragged['schedule']}), {ragged['default_schedule_tracks_per_s']:.4g} default",
+              file=sys.stderr, flush=True)
     adj = None
     if not args.no_adjoint:
         adj = adjoint_leg(max(args.steps // 2, 3), 1)
@@ -275,6 +321,7 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
         "error_vs_oracle": errors,
+        "ragged": ragged,
         "adjoint": adj,
     }
     if "fp64" in rates and "mixed" in rates:

@@ -98,6 +98,20 @@ class kf_tracks(C.Structure):
     ]
 
 
+class kf_schedule(C.Structure):
+    _fields_ = [
+        ("sort_by_length", C.c_int32),
+        ("persistent", C.c_int32),
+        ("waves_per_cu", C.c_int32),
+        ("chunks_per_grab", C.c_int32),
+        ("lanes_per_track", C.c_int32),
+        ("graph", C.c_int32),
+    ]
+
+    def as_dict(self):
+        return {k: int(getattr(self, k)) for k, _ in self._fields_}
+
+
 class kf_results(C.Structure):
     _fields_ = [
         ("params", C.c_void_p),

@@ -35,6 +35,18 @@ struct kf_ctx {
   bool has_material = false;  // a plane of the table carries material
   hipEvent_t ev0 = nullptr, evm = nullptr, ev1 = nullptr;
   bool timed = false;
+  bool phase_timed = false;  // evm was recorded (not under graph replay)
+  // scheduling (kf_schedule) and its buffers: track order, sort counters,
+  // work-queue counters
+  kf_schedule sched{};
+  int n_cu = 0;
+  void *sched_buf = nullptr;
+  size_t sched_bytes = 0;
+  // graph replay: the captured launch sequence and the arguments it holds
+  hipStream_t cap_stream = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  std::vector<char> gkey;
 };
 
 namespace {
@@ -133,6 +145,9 @@ int kf_ctx_create(int device, kf_ctx **out) {
   if (e == hipSuccess) e = hipEventCreate(&ctx->ev0);
   if (e == hipSuccess) e = hipEventCreate(&ctx->ev1);
   if (e == hipSuccess) e = hipEventCreate(&ctx->evm);
+  if (e == hipSuccess) e = hipDeviceGetAttribute(&ctx->n_cu, hipDeviceAttributeMultiprocessorCount, device);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->cap_stream, hipStreamNonBlocking);
+  kf_schedule_default(&ctx->sched);
   if (e != hipSuccess) {
     fail(nullptr, KF_E_HIP, "kf_ctx_create: %s", hipGetErrorString(e));
     delete ctx;
@@ -154,6 +169,10 @@ int kf_ctx_destroy(kf_ctx *ctx) {
   if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
   if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
   if (ctx->evm) (void)hipEventDestroy(ctx->evm);
+  if (ctx->gexec) (void)hipGraphExecDestroy(ctx->gexec);
+  if (ctx->graph) (void)hipGraphDestroy(ctx->graph);
+  if (ctx->cap_stream) (void)hipStreamDestroy(ctx->cap_stream);
+  if (ctx->sched_buf) (void)hipFree(ctx->sched_buf);
   delete ctx;
   return KF_OK;
 }
@@ -356,11 +375,142 @@ int kf_fit(kf_ctx *ctx, const kf_tracks *tr, const kf_options *opt, const kf_res
   hipStream_t s = (hipStream_t)stream;
   rc = setupStores(ctx, opt->precision, res->track_states != nullptr, false, s, a);
   if (rc != KF_OK) return rc;
-  KF_HIP(ctx, hipEventRecord(ctx->ev0, s));
-  hipError_t e = kfd_launch_fit(&a, opt->precision, s, res->track_states, ctx->evm);
+  const kf_schedule &sc = ctx->sched;
+  KfdLaunch L{};
+  if (sc.sort_by_length || sc.persistent) {
+    const size_t order_bytes = ((size_t)a.n * sizeof(int32_t) + 255) / 256 * 256;
+    rc = grow(ctx, &ctx->sched_buf, &ctx->sched_bytes, order_bytes + 514 * sizeof(uint32_t), s,
+              "schedule buffers");
+    if (rc != KF_OK) return rc;
+    L.order = (int32_t *)ctx->sched_buf;
+    L.sort_scratch = (uint32_t *)((char *)ctx->sched_buf + order_bytes);
+    L.sort_by_length = sc.sort_by_length;
+    a.order = sc.sort_by_length ? L.order : nullptr;
+    a.queue = L.sort_scratch + 512;
+    if (sc.persistent) {
+      const int waves = sc.waves_per_cu * ctx->n_cu;
+      L.persistent_blocks = (waves * 64 + KF_BLOCK - 1) / KF_BLOCK;
+      a.persistent = 1;
+      a.chunks_per_grab = sc.chunks_per_grab;
+    }
+  }
+  hipError_t e = hipSuccess;
+  if (sc.graph) {
+    // key: every argument the captured launches hold
+    std::vector<char> key(sizeof a + sizeof L + sizeof(int) + sizeof(void *));
+    char *k = key.data();
+    std::memcpy(k, &a, sizeof a);
+    std::memcpy(k + sizeof a, &L, sizeof L);
+    std::memcpy(k + sizeof a + sizeof L, &opt->precision, sizeof(int));
+    std::memcpy(k + sizeof a + sizeof L + sizeof(int), &res->track_states, sizeof(void *));
+    if (!ctx->gexec || key != ctx->gkey) {
+      if (ctx->gexec) (void)hipGraphExecDestroy(ctx->gexec);
+      if (ctx->graph) (void)hipGraphDestroy(ctx->graph);
+      ctx->gexec = nullptr;
+      ctx->graph = nullptr;
+      ctx->gkey.clear();
+      KF_HIP(ctx, hipStreamSynchronize(s));
+      KF_HIP(ctx, hipStreamBeginCapture(ctx->cap_stream, hipStreamCaptureModeThreadLocal));
+      e = kfd_launch_fit(&a, opt->precision, ctx->cap_stream, res->track_states, nullptr, &L);
+      hipGraph_t g = nullptr;
+      const hipError_t e2 = hipStreamEndCapture(ctx->cap_stream, &g);
+      if (e == hipSuccess) e = e2;
+      if (e != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        return fail(ctx, KF_E_HIP, "kf_fit: graph capture failed: %s", hipGetErrorString(e));
+      }
+      ctx->graph = g;
+      KF_HIP(ctx, hipGraphInstantiate(&ctx->gexec, g, nullptr, nullptr, 0));
+      ctx->gkey = key;
+    }
+    KF_HIP(ctx, hipEventRecord(ctx->ev0, s));
+    e = hipGraphLaunch(ctx->gexec, s);
+    ctx->phase_timed = false;
+  } else {
+    KF_HIP(ctx, hipEventRecord(ctx->ev0, s));
+    e = kfd_launch_fit(&a, opt->precision, s, res->track_states, ctx->evm, &L);
+    ctx->phase_timed = true;
+  }
   if (e != hipSuccess) return fail(ctx, KF_E_HIP, "kf_fit: launch failed: %s", hipGetErrorString(e));
   KF_HIP(ctx, hipEventRecord(ctx->ev1, s));
   ctx->timed = true;
+  return KF_OK;
+}
+
+void kf_schedule_default(kf_schedule *s) {
+  if (!s) return;
+  s->sort_by_length = 0;
+  s->persistent = 0;
+  s->waves_per_cu = 4;
+  s->chunks_per_grab = 1;
+  s->lanes_per_track = 1;
+  s->graph = 0;
+}
+
+int kf_set_schedule(kf_ctx *ctx, const kf_schedule *s) {
+  if (!ctx) return fail(nullptr, KF_E_INVALID, "kf_set_schedule: ctx is NULL");
+  if (!s) return fail(ctx, KF_E_INVALID, "kf_set_schedule: schedule is NULL");
+  if (s->lanes_per_track != 1)
+    return fail(ctx, KF_E_INVALID, "kf_set_schedule: lanes_per_track must be 1 (one track per lane)");
+  if (s->persistent && (s->waves_per_cu < 1 || s->waves_per_cu > 64 || s->chunks_per_grab < 1 ||
+                        s->chunks_per_grab > 1024))
+    return fail(ctx, KF_E_INVALID, "kf_set_schedule: waves_per_cu in [1, 64], chunks_per_grab in [1, 1024]");
+  ctx->sched = *s;
+  return KF_OK;
+}
+
+int kf_get_schedule(const kf_ctx *ctx, kf_schedule *s) {
+  if (!ctx || !s) return KF_E_INVALID;
+  *s = ctx->sched;
+  return KF_OK;
+}
+
+int kf_autotune(kf_ctx *ctx, const kf_tracks *tracks, const kf_options *opt, const kf_results *res,
+                void *stream, int32_t repeats, kf_schedule *best, double *best_ms) {
+  if (!ctx) return fail(nullptr, KF_E_INVALID, "kf_autotune: ctx is NULL");
+  if (!tracks || !opt || !res) return fail(ctx, KF_E_INVALID, "kf_autotune: NULL argument");
+  if (repeats < 1) repeats = 1;
+  std::vector<kf_schedule> cand;
+  for (int sort = 0; sort <= 1; ++sort) {
+    kf_schedule c;
+    kf_schedule_default(&c);
+    c.sort_by_length = sort;
+    c.graph = 1;
+    cand.push_back(c);
+    for (int w : {2, 4, 8})
+      for (int g : {1, 4}) {
+        c.persistent = 1;
+        c.waves_per_cu = w;
+        c.chunks_per_grab = g;
+        cand.push_back(c);
+      }
+  }
+  const kf_schedule saved = ctx->sched;
+  double best_t = 1e300;
+  kf_schedule best_s = saved;
+  for (const kf_schedule &c : cand) {
+    int rc = kf_set_schedule(ctx, &c);
+    if (rc == KF_OK) rc = kf_fit(ctx, tracks, opt, res, stream);  // warm-up / capture
+    double t = 1e300;
+    for (int r = 0; rc == KF_OK && r < repeats; ++r) {
+      rc = kf_fit(ctx, tracks, opt, res, stream);
+      if (rc == KF_OK) {
+        const double ms = kf_last_kernel_ms(ctx);
+        if (ms >= 0.0 && ms < t) t = ms;
+      }
+    }
+    if (rc != KF_OK) {
+      ctx->sched = saved;
+      return rc;
+    }
+    if (t < best_t) {
+      best_t = t;
+      best_s = c;
+    }
+  }
+  ctx->sched = best_s;
+  if (best) *best = best_s;
+  if (best_ms) *best_ms = best_t;
   return KF_OK;
 }
 
@@ -420,7 +570,9 @@ int kf_chi2_gradient(kf_ctx *ctx, const kf_tracks *tr, const kf_options *opt, do
   p += al(b_flags);
   double *sums = (double *)p;
   KF_HIP(ctx, hipEventRecord(ctx->ev0, s));
-  hipError_t e = (n > 0) ? kfd_launch_fit(&a, KF_PRECISION_FP64, s, nullptr, ctx->evm) : hipSuccess;
+  hipError_t e =
+      (n > 0) ? kfd_launch_fit(&a, KF_PRECISION_FP64, s, nullptr, ctx->evm, nullptr) : hipSuccess;
+  ctx->phase_timed = true;
   if (e == hipSuccess && n == 0) e = hipEventRecord(ctx->evm, s);
   if (e == hipSuccess)
     e = kfd_launch_adjoint(&a, &g, sums, grad_planes, grad_field_scale, chi2_sum, s);
@@ -445,7 +597,7 @@ double kf_last_kernel_ms(const kf_ctx *ctx) {
 int kf_last_phase_ms(const kf_ctx *ctx, double *forward_ms, double *backward_ms) {
   if (!ctx || !forward_ms || !backward_ms) return KF_E_INVALID;
   *forward_ms = *backward_ms = -1.0;
-  if (!ctx->timed) return KF_E_STATE;
+  if (!ctx->timed || !ctx->phase_timed) return KF_E_STATE;
   if (hipEventSynchronize(ctx->ev1) != hipSuccess) return KF_E_HIP;
   float f = 0.0f, b = 0.0f;
   if (hipEventElapsedTime(&f, ctx->ev0, ctx->evm) != hipSuccess ||

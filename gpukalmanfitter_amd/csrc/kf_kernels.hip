@@ -25,57 +25,153 @@ struct ChunkBases {
   }
 };
 
-template <typename S>
-__global__ void __launch_bounds__(KF_BLOCK) kf_forward_kernel(FitArgs args) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const ChunkBases<S> cb(args, i);
-  if (i >= args.n) return;
-  TrackFit<S, true> tf(args, i, cb.d, cb.s, cb.f, cb.c, (uint32_t)(i & 63));
-  tf.runForward();
+// Fits the track at position pos (pass FWD of the two kernels).
+template <typename S, bool FWD>
+KFD_INLINE void fitPosition(const FitArgs &a, int64_t pos) {
+  const ChunkBases<S> cb(a, pos);
+  if (pos >= a.n) return;
+  const int64_t i = a.order ? (int64_t)a.order[pos] : pos;
+  TrackFit<S, FWD> tf(a, i, cb.d, cb.s, cb.f, cb.c, (uint32_t)(pos & 63));
+  if constexpr (FWD)
+    tf.runForward();
+  else
+    tf.runBackward();
 }
 
-template <typename S>
-__global__ void __launch_bounds__(KF_BLOCK) kf_backward_kernel(FitArgs args) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const ChunkBases<S> cb(args, i);
-  if (i >= args.n) return;
-  TrackFit<S, false> tf(args, i, cb.d, cb.s, cb.f, cb.c, (uint32_t)(i & 63));
-  tf.runBackward();
+// One thread per position, or (persistent) resident waves that take
+// chunks_per_grab 64-position chunks at a time from a work-queue counter
+// until every chunk is taken; the grab is one vector atomic by lane 0.
+template <typename S, bool FWD>
+__global__ void __launch_bounds__(KF_BLOCK) kf_fit_kernel(FitArgs args) {
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t nchunks = (uint32_t)((args.n + 63) >> 6);
+  const uint32_t grab = (uint32_t)args.chunks_per_grab;
+  uint32_t *q = args.queue + (FWD ? 0 : 1);
+  // one-shot grid: this wave's chunk; persistent: chunks come from the queue
+  uint32_t c = blockIdx.x * (KF_BLOCK / 64) + (threadIdx.x >> 6);
+  uint32_t c1 = c + 1;
+  for (;;) {
+    if (args.persistent) {
+      uint32_t c0 = 0;
+      if (lane == 0) c0 = atomicAdd(q, grab);
+      c0 = (uint32_t)__shfl((int)c0, 0, 64);
+      if (c0 >= nchunks) break;
+      c = c0;
+      c1 = (c0 + grab < nchunks) ? c0 + grab : nchunks;
+    }
+    for (; c < c1; ++c) fitPosition<S, FWD>(args, (int64_t)c * 64 + lane);
+    if (!args.persistent) break;
+  }
 }
 
 // Copies the public track-state records into kf_results.track_states
 // ([state][KF_TS_STRIDE][n]).
 __global__ void __launch_bounds__(256) kf_export_states(FitArgs args, double *out) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= args.n) return;
+  const int64_t pos = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (pos >= args.n) return;
+  const int64_t i = args.order ? (int64_t)args.order[pos] : pos;
   const int nm = args.out_nmeas[i];
-  const double *base = args.tsf + (size_t)(i >> 6) * (size_t)args.max_states * KF_TS_STRIDE * 64u;
-  const uint32_t lane = (uint32_t)(i & 63);
+  const double *base =
+      args.tsf + (size_t)(pos >> 6) * (size_t)args.max_states * KF_TS_STRIDE * 64u;
+  const uint32_t lane = (uint32_t)(pos & 63);
   for (int k = 0; k < nm && k < args.max_states; ++k)
     for (int f = 0; f < KF_TS_STRIDE; ++f)
       out[((size_t)k * KF_TS_STRIDE + f) * (size_t)args.n + i] =
           base[((uint32_t)k * KF_TS_STRIDE + (uint32_t)f) * 64u + lane];
 }
 
+// ------------------------------------------------------- length sort ----
+// Counting sort of the tracks by decreasing sequence length (lengths >= 255
+// share the top bin): tracks of one wave then take similar numbers of steps,
+// and a persistent grid starts on the longest.  Per-block LDS histograms keep
+// the global atomics to one per block and bin.
+constexpr int kLenBins = 256;
+
+KFD_INLINE int lengthBin(const FitArgs &a, int64_t i) {
+  int l = a.seq_count[i];
+  return l < 0 ? 0 : (l >= kLenBins ? kLenBins - 1 : l);
+}
+
+// zeroes the histogram and bin offsets and the work-queue counters
+__global__ void __launch_bounds__(256) kf_sched_reset(uint32_t *scratch, uint32_t *queue) {
+  scratch[threadIdx.x] = 0u;
+  scratch[kLenBins + threadIdx.x] = 0u;
+  if (queue && threadIdx.x < 2) queue[threadIdx.x] = 0u;
+}
+
+__global__ void __launch_bounds__(256) kf_len_hist(FitArgs a, uint32_t *hist) {
+  __shared__ uint32_t h[kLenBins];
+  h[threadIdx.x] = 0u;
+  __syncthreads();
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < a.n) atomicAdd(&h[lengthBin(a, i)], 1u);
+  __syncthreads();
+  if (h[threadIdx.x]) atomicAdd(&hist[threadIdx.x], h[threadIdx.x]);
+}
+
+// off[len] = number of tracks longer than len (one block of 256)
+__global__ void __launch_bounds__(256) kf_len_scan(uint32_t *hist, uint32_t *off) {
+  __shared__ uint32_t s[kLenBins];
+  const int t = threadIdx.x;
+  s[t] = hist[kLenBins - 1 - t];
+  __syncthreads();
+  for (int w = 1; w < kLenBins; w <<= 1) {
+    const uint32_t v = (t >= w) ? s[t - w] : 0u;
+    __syncthreads();
+    s[t] += v;
+    __syncthreads();
+  }
+  off[kLenBins - 1 - t] = s[t] - hist[kLenBins - 1 - t];
+}
+
+__global__ void __launch_bounds__(256) kf_len_scatter(FitArgs a, uint32_t *off, int32_t *order) {
+  __shared__ uint32_t cnt[kLenBins], base[kLenBins];
+  cnt[threadIdx.x] = 0u;
+  __syncthreads();
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int bin = (i < a.n) ? lengthBin(a, i) : 0;
+  uint32_t local = 0;
+  if (i < a.n) local = atomicAdd(&cnt[bin], 1u);
+  __syncthreads();
+  if (cnt[threadIdx.x]) base[threadIdx.x] = atomicAdd(&off[threadIdx.x], cnt[threadIdx.x]);
+  __syncthreads();
+  if (i < a.n) order[base[bin] + local] = (int32_t)i;
+}
+
 }  // namespace kfd
 
 extern "C" hipError_t kfd_launch_fit(const kfd::FitArgs *args, int precision, hipStream_t stream,
-                                     double *track_states_out, hipEvent_t between) {
-  const int64_t blocks = (args->n + KF_BLOCK - 1) / KF_BLOCK;
-  if (blocks <= 0) return hipSuccess;
+                                     double *track_states_out, hipEvent_t between,
+                                     const KfdLaunch *launch) {
+  if (args->n <= 0) return hipSuccess;
+  const bool sort = launch && launch->sort_by_length;
+  const bool pers = launch && launch->persistent_blocks > 0;
+  if (sort || pers) {
+    hipLaunchKernelGGL(kfd::kf_sched_reset, dim3(1), dim3(256), 0, stream, launch->sort_scratch,
+                       args->queue);
+  }
+  if (sort) {
+    const unsigned sb = (unsigned)((args->n + 255) / 256);
+    uint32_t *hist = launch->sort_scratch, *off = launch->sort_scratch + kfd::kLenBins;
+    hipLaunchKernelGGL(kfd::kf_len_hist, dim3(sb), dim3(256), 0, stream, *args, hist);
+    hipLaunchKernelGGL(kfd::kf_len_scan, dim3(1), dim3(256), 0, stream, hist, off);
+    hipLaunchKernelGGL(kfd::kf_len_scatter, dim3(sb), dim3(256), 0, stream, *args, off,
+                       launch->order);
+  }
+  const int64_t blocks = pers ? launch->persistent_blocks : (args->n + KF_BLOCK - 1) / KF_BLOCK;
   const dim3 grid((unsigned)blocks), block(KF_BLOCK);
   if (precision == KF_PRECISION_FP64)
-    hipLaunchKernelGGL(kfd::kf_forward_kernel<double>, grid, block, 0, stream, *args);
+    hipLaunchKernelGGL((kfd::kf_fit_kernel<double, true>), grid, block, 0, stream, *args);
   else
-    hipLaunchKernelGGL(kfd::kf_forward_kernel<float>, grid, block, 0, stream, *args);
+    hipLaunchKernelGGL((kfd::kf_fit_kernel<float, true>), grid, block, 0, stream, *args);
   if (between) {
     const hipError_t e = hipEventRecord(between, stream);
     if (e != hipSuccess) return e;
   }
   if (precision == KF_PRECISION_FP64)
-    hipLaunchKernelGGL(kfd::kf_backward_kernel<double>, grid, block, 0, stream, *args);
+    hipLaunchKernelGGL((kfd::kf_fit_kernel<double, false>), grid, block, 0, stream, *args);
   else
-    hipLaunchKernelGGL(kfd::kf_backward_kernel<float>, grid, block, 0, stream, *args);
+    hipLaunchKernelGGL((kfd::kf_fit_kernel<float, false>), grid, block, 0, stream, *args);
   if (track_states_out) {
     const int64_t eb = (args->n + 255) / 256;
     hipLaunchKernelGGL(kfd::kf_export_states, dim3((unsigned)eb), dim3(256), 0, stream, *args,

@@ -84,13 +84,19 @@ struct FitArgs {
   // forward -> backward kernel hand-over, [field][n]
   int32_t *carry_i;  // 6 int fields
   double *carry_d;   // 30 double fields
+  // Scheduling.  Tracks are processed at positions 0..n-1; the track at
+  // position p is order[p] (identity when NULL) and its store slot is that
+  // of position p.  persistent = 1: a grid of resident waves pulls
+  // chunks_per_grab 64-position chunks at a time from queue[0] (forward) and
+  // queue[1] (backward), both zeroed before the launches.
+  const int32_t *order;
+  int32_t persistent;
+  int32_t chunks_per_grab;
+  uint32_t *queue;
 };
 
 }  // namespace kfd
 
-// Launches the forward kernel, records `between` (if not NULL), launches the
-// smoother/back-propagation kernel, then the track-state export if
-// track_states_out is not NULL.
 // Step-stack depth per track of the chi2 adjoint: a segment between two
 // measured surfaces may take this many RK steps.
 #define KF_ADJ_MAX_SEGMENT_STEPS 48
@@ -104,5 +110,19 @@ extern "C" hipError_t kfd_launch_adjoint(const kfd::FitArgs *args, const kfd::Ad
                                          double *sums, double *grad_planes, double *grad_scale,
                                          double *chi2, hipStream_t stream);
 
+// Launch shape of kfd_launch_fit.
+struct KfdLaunch {
+  int32_t sort_by_length;  // order positions by decreasing sequence length first
+  int32_t persistent_blocks;  // > 0: persistent grid of this many blocks
+  uint32_t *sort_scratch;  // 256 + 256 counters (length histogram, bin offsets)
+  int32_t *order;          // [n] positions -> tracks (when sorting)
+};
+
+// Launches the fit: [length sort] -> forward kernel -> record `between` (if
+// not NULL) -> smoother/back-propagation kernel -> track-state export (if
+// track_states_out is not NULL).  `launch` may be NULL (one thread per
+// track, identity order).  With sorting or a persistent grid, args->order,
+// args->queue and args->chunks_per_grab must be set.
 extern "C" hipError_t kfd_launch_fit(const kfd::FitArgs *args, int precision, hipStream_t stream,
-                                     double *track_states_out, hipEvent_t between);
+                                     double *track_states_out, hipEvent_t between,
+                                     const KfdLaunch *launch);

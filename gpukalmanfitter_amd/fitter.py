@@ -57,6 +57,16 @@ def _lib():
         lib.kf_chi2_gradient.argtypes = [P, C.POINTER(abi.kf_tracks), C.POINTER(abi.kf_options),
                                          P, P, P, P, P, P]
         lib.kf_chi2_gradient.restype = C.c_int
+        lib.kf_schedule_default.argtypes = [C.POINTER(abi.kf_schedule)]
+        lib.kf_schedule_default.restype = None
+        lib.kf_set_schedule.argtypes = [P, C.POINTER(abi.kf_schedule)]
+        lib.kf_set_schedule.restype = C.c_int
+        lib.kf_get_schedule.argtypes = [P, C.POINTER(abi.kf_schedule)]
+        lib.kf_get_schedule.restype = C.c_int
+        lib.kf_autotune.argtypes = [P, C.POINTER(abi.kf_tracks), C.POINTER(abi.kf_options),
+                                    C.POINTER(abi.kf_results), P, C.c_int32,
+                                    C.POINTER(abi.kf_schedule), C.POINTER(C.c_double)]
+        lib.kf_autotune.restype = C.c_int
         lib._kf_typed = True
     return lib
 
@@ -208,6 +218,11 @@ class DeviceBatch:
             setattr(r, name, tensor.data_ptr())
         return r
 
+    def clear_results(self):
+        """Zero-fills every output buffer (fill kernels on the current stream)."""
+        for v in self.out.values():
+            v.zero_()
+
     def results_numpy(self):
         """Copies the outputs to host numpy arrays (synchronising)."""
         res = {k: v.cpu().numpy() for k, v in self.out.items()}
@@ -278,6 +293,35 @@ class KalmanFitter:
         s = stream if stream is not None else torch.cuda.current_stream(dev_batch.device)
         self._check(self._lib.kf_fit(self._ctx, C.byref(tracks), C.byref(opts), C.byref(results),
                                      C.c_void_p(s.cuda_stream)))
+
+    def set_schedule(self, **fields):
+        """Sets kf_schedule fields (unnamed ones keep the library defaults)."""
+        s = abi.kf_schedule()
+        self._lib.kf_schedule_default(C.byref(s))
+        for k, v in fields.items():
+            setattr(s, k, int(v))
+        self._check(self._lib.kf_set_schedule(self._ctx, C.byref(s)))
+
+    def schedule(self):
+        s = abi.kf_schedule()
+        self._check(self._lib.kf_get_schedule(self._ctx, C.byref(s)))
+        return s.as_dict()
+
+    def autotune(self, dev_batch, options=None, stream=None, repeats=2):
+        """Runs the library's autotuner on this batch; returns (schedule dict, ms)."""
+        torch = _torch()
+        opts = (options or KalmanFitterOptions()).to_c()
+        if opts.max_states == 0:
+            opts.max_states = dev_batch.max_states
+        tracks = dev_batch.c_tracks()
+        results = dev_batch.c_results()
+        s = stream if stream is not None else torch.cuda.current_stream(dev_batch.device)
+        best = abi.kf_schedule()
+        ms = C.c_double()
+        self._check(self._lib.kf_autotune(self._ctx, C.byref(tracks), C.byref(opts),
+                                          C.byref(results), C.c_void_p(s.cuda_stream),
+                                          int(repeats), C.byref(best), C.byref(ms)))
+        return best.as_dict(), ms.value
 
     def set_geometry(self, planes):
         """Replaces the plane table (same number of planes or not)."""

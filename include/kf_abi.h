@@ -156,6 +156,19 @@ typedef struct kf_results {
   double *track_states; /* optional [max_states][KF_TS_STRIDE][n_tracks] */
 } kf_results;
 
+/* How kf_fit lays the batch onto the GPU (this build's; the reference
+ * launches one thread per track on a fixed grid, Run/GPU/Kernels.cu:6-55). */
+typedef struct kf_schedule {
+  int32_t sort_by_length;  /* fit tracks in order of decreasing sequence length, so a
+                              wave's tracks take similar numbers of steps (default 0) */
+  int32_t persistent;      /* 1: resident waves pull 64-track chunks from a work queue */
+  int32_t waves_per_cu;    /* persistent grid size, waves per compute unit (default 4) */
+  int32_t chunks_per_grab; /* work-queue granularity, 64-track chunks per grab (default 1) */
+  int32_t lanes_per_track; /* lanes cooperating on one track: 1 (the only variant) */
+  int32_t graph;           /* 1: capture the launch sequence as a hipGraph on the first
+                              call and replay it while the arguments are unchanged */
+} kf_schedule;
+
 typedef struct kf_ctx kf_ctx;
 
 int kf_abi_version(void);
@@ -211,6 +224,22 @@ int kf_fit(kf_ctx *ctx, const kf_tracks *tracks, const kf_options *opt,
 int kf_chi2_gradient(kf_ctx *ctx, const kf_tracks *tracks, const kf_options *opt,
                      double *grad_planes, double *grad_field_scale, double *chi2_sum,
                      double *grad_raw, uint32_t *track_flags, void *stream);
+
+/* Schedule of later kf_fit calls on ctx (kf_schedule_default: one thread per
+ * track in input order, no graph). */
+void kf_schedule_default(kf_schedule *s);
+int kf_set_schedule(kf_ctx *ctx, const kf_schedule *s);
+int kf_get_schedule(const kf_ctx *ctx, kf_schedule *s);
+
+/* Autotuner: fits the batch with each candidate schedule (sort on/off,
+ * one-shot grid or persistent work queue with 2/4/8 waves per CU and 1/4
+ * chunks per grab, graph replay), times every candidate on `stream` with HIP
+ * events (best of `repeats` fits), keeps the fastest as ctx's schedule and
+ * returns it with its time.  The results buffers hold the last fit.
+ * lanes_per_track is 1 in every candidate. */
+int kf_autotune(kf_ctx *ctx, const kf_tracks *tracks, const kf_options *opt,
+                const kf_results *res, void *stream, int32_t repeats, kf_schedule *best,
+                double *best_ms);
 
 /* Duration of the last fit kernel on ctx in milliseconds (HIP events recorded
  * on the launch stream); < 0 if unknown. */

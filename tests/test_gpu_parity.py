@@ -311,3 +311,54 @@ def test_mixed_full_size_config2_pulls(torch_cuda):
     assert np.array_equal(g["status"], o.status)
     pd, cd, c2 = compare_fit(g, o)
     assert pd <= TOL_MIXED_PARAMS and cd <= TOL_MIXED_COV, (pd, cd)
+
+
+# ------------------------------------------------------------- scheduling --
+SCHEDULES = [dict(sort_by_length=1), dict(persistent=1, waves_per_cu=4, chunks_per_grab=1),
+             dict(sort_by_length=1, persistent=1, waves_per_cu=8, chunks_per_grab=4, graph=1),
+             dict(graph=1)]
+
+
+@pytest.mark.parametrize("sched", SCHEDULES)
+def test_schedules_give_identical_results(torch_cuda, sched):
+    """Length sort, persistent work queue and graph replay change where a
+    track runs, never its result: bitwise equal to the default schedule, and
+    the oracle on a ragged batch with holes (grid field)."""
+    from gpukalmanfitter_amd.fitter import DeviceBatch, KalmanFitter, KalmanFitterOptions
+    b = G.generate(3, n_tracks=5000)
+    kf = KalmanFitter(b.planes, b.field)
+    dev = DeviceBatch(b, track_states=True)
+    opts = KalmanFitterOptions(**W)
+    kf.fit_batch(dev, opts)
+    ref = dev.results_numpy()
+    kf.set_schedule(**sched)
+    for _ in range(3):  # graph: capture, then two replays
+        dev.clear_results()
+        kf.fit_batch(dev, opts)
+        got = dev.results_numpy()
+        for k in ("params", "cov", "chi2", "status", "n_measured", "track_states"):
+            assert np.array_equal(got[k], ref[k]), k
+    kf.close()
+    o, _, _, _ = oracle.fit(b, c_options(**W))
+    check_parity(ref, o)
+
+
+def test_autotuner_on_ragged_batch(torch_cuda):
+    """The autotuner times its candidate schedules on the batch and keeps the
+    fastest; the kept schedule reproduces the default results."""
+    from gpukalmanfitter_amd.fitter import DeviceBatch, KalmanFitter, KalmanFitterOptions
+    b = G.generate(3, n_tracks=200000)
+    kf = KalmanFitter(b.planes, b.field)
+    dev = DeviceBatch(b)
+    opts = KalmanFitterOptions(precision="mixed", **W)
+    kf.fit_batch(dev, opts)
+    ref = dev.results_numpy()
+    best, ms = kf.autotune(dev, opts)
+    print("autotuned schedule", best, f"{ms:.3f} ms")
+    assert best["lanes_per_track"] == 1 and ms > 0
+    assert kf.schedule() == best
+    kf.fit_batch(dev, opts)
+    got = dev.results_numpy()
+    for k in ("params", "cov", "status"):
+        assert np.array_equal(got[k], ref[k]), k
+    kf.close()
