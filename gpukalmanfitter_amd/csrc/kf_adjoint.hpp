@@ -24,8 +24,10 @@
 // Derivative conventions: discrete decisions (step accept/reject, surface
 // reached, hole) are held fixed; a step whose length was set by the
 // navigator (distance to the target plane) is differentiated through that
-// distance, a step set by the error control is held fixed.  Material
-// interaction is not differentiated (the entry point refuses material).
+// distance, a step set by the error control is held fixed.  The material
+// update (energy loss and the scattering / straggling variances) is
+// differentiated with respect to the momentum, the direction and the plane
+// normal (path length through the slab).
 #pragma once
 
 #include "kf_trackfit.hpp"
@@ -770,10 +772,10 @@ struct PlaneFwd {
 
 // Backward of the free state made from the filtered parameters on the plane
 // (updateFromBound): pos = c + u0 xf0 + u1 xf1, dir = dir(phi, theta),
-// q/p = q |xf4|.  Accumulates into xfb, cb, Rb.
-KFD_INLINE void freeFromBoundVJP(const PlaneFwd &P, double q, const double posb[3],
-                                 const double db[3], double qopb, double xfb[5], double cb[3],
-                                 double Rb[3][3]) {
+// p = 1 / |xf4|.  pBar is the adjoint of that momentum.  Accumulates into
+// xfb, cb, Rb.
+KFD_INLINE void freeFromBoundVJP(const PlaneFwd &P, const double posb[3], const double db[3],
+                                 double pBar, double xfb[5], double cb[3], double Rb[3][3]) {
   const double ph = P.xf[2], th = P.xf[3];
   const double cp = cos(ph), sp = sin(ph), ct = cos(th), st = sin(th);
 #pragma unroll
@@ -786,7 +788,8 @@ KFD_INLINE void freeFromBoundVJP(const PlaneFwd &P, double q, const double posb[
   }
   xfb[2] += db[0] * (-sp * st) + db[1] * (cp * st);
   xfb[3] += db[0] * (cp * ct) + db[1] * (sp * ct) + db[2] * (-st);
-  xfb[4] += qopb * q * ((P.xf[4] < 0.0) ? -1.0 : 1.0);
+  const double x4 = P.xf[4];
+  xfb[4] += pBar * (-((x4 < 0.0) ? -1.0 : 1.0) / (x4 * x4));
 }
 
 // Backward of the Jacobian re-initialisation on a plane (PlaneSurface.ipp:
@@ -808,6 +811,130 @@ KFD_INLINE void jacInitVJP(const double d[3], const double Ab[3][5], const doubl
            Gb[1][1] * (d2 / rho - d2 * d1 * d1 / r3) + Gb[2][1] * (-d1 / rho);
   db[2] += Gb[0][1] * (d0 / rho) + Gb[1][1] * (d1 / rho);
 }
+
+// ---------------------------------------------------------------------------
+// Material update on a plane (TrackFit::material in double: Bethe mean loss,
+// Highland scattering, Landau straggling in q/p), with its backward pass.
+// Maps (p, d, plane normal) to the new momentum and the three variances
+// added to cov(2,2), cov(3,3), cov(4,4).
+struct MaterialFwd {
+  bool on;
+  double p, m, sgn, c, pcor, q2b2, dq2b2, eps, depsP, depsC, running, drunP, Eloss;
+  double E0, nextE, pOut;
+  double tt, theta0, rho, dn, sigPhi, vPhi, vTheta, sq, vQop;
+  bool ms, el, dens;
+
+  KFD_INLINE void run(const FitArgs &a, const DevPlane &pl, double pIn, const double d[3]) {
+    on = pl.has_material && (a.ms || a.eloss);
+    ms = a.ms;
+    el = a.eloss;
+    p = pIn;
+    m = a.mass;
+    pOut = p;
+    vPhi = vTheta = vQop = 0.0;
+    if (!on) return;
+    const double nr[3] = {pl.r[2], pl.r[5], pl.r[8]};
+    const double nd = nr[0] * d[0] + nr[1] * d[1] + nr[2] * d[2];
+    sgn = (nd < 0.0) ? -1.0 : 1.0;
+    c = fabs(nd);
+    pcor = 1.0 / c;
+    const double thick = pl.thickness * pcor, xOverX0 = pl.thicknessInX0 * pcor;
+    q2b2 = 1.0 + (m / p) * (m / p);
+    dq2b2 = -2.0 * m * m / (p * p * p);
+    const double K = 0.5 * a.kK * pl.molarElectronDensity;
+    eps = K * thick * q2b2;
+    depsP = K * thick * dq2b2;
+    depsC = K * pl.thickness * q2b2;  // d eps / d pcor
+    running = 0.0;
+    drunP = 0.0;
+    Eloss = 0.0;
+    if (el) {
+      const double I = pl.meanExcitation, me = a.kMe, bg = p / m, gamma = sqrt(1.0 + bg * bg);
+      const double mfrac = me / m, den = 1.0 + 2.0 * gamma * mfrac + mfrac * mfrac;
+      const double u = 2.0 * me * bg * bg, wmax = u / den;
+      const double beta2 = 1.0 / (1.0 + (m / p) * (m / p));
+      dens = !(bg < 10.0);
+      const double dhalf = dens ? log(bg) + log(a.kPlasma * sqrt(pl.molarElectronDensity) / I) - 0.5 : 0.0;
+      running = 0.5 * log(u / I) + 0.5 * log(wmax / I) - beta2 - dhalf;
+      const double dgamma = bg / (gamma * m);
+      const double dbeta2 = 2.0 * p * m * m / ((p * p + m * m) * (p * p + m * m));
+      drunP = 1.0 / p + 0.5 * (2.0 / p - 2.0 * mfrac * dgamma / den) - dbeta2 - (dens ? 1.0 / p : 0.0);
+      Eloss = eps * running;
+    }
+    if (ms) {
+      tt = sqrt(xOverX0 * q2b2);
+      theta0 = a.kHighland * (1.0 / p) * tt * (1.0 + a.kHighlandLog * log(tt));
+      rho = sqrt(d[0] * d[0] + d[1] * d[1]);
+      dn = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+      sigPhi = theta0 * (dn / rho);
+      vPhi = sigPhi * sigPhi;
+      vTheta = theta0 * theta0;
+    }
+    if (el) {
+      sq = sqrt(q2b2) * (1.0 / (p * p)) * (4.0 * eps / a.kLandauConv);
+      vQop = sq * sq;
+    }
+    E0 = sqrt(m * m + p * p);
+    nextE = E0 - Eloss;
+    pOut = (m < nextE) ? sqrt(nextE * nextE - m * m) : 0.0;
+  }
+
+  // In: pOutb, the adjoints of the three variances.  Out: the adjoint of the
+  // incoming momentum; accumulates into db and nb (plane normal).
+  KFD_INLINE double vjp(const FitArgs &a, const DevPlane &pl, const double d[3], double pOutb,
+                        double vPhib, double vThetab, double vQopb, double db[3],
+                        double nb[3]) const {
+    if (!on) return pOutb;
+    double pb = 0.0, pcorb = 0.0;
+    // p' = sqrt(nextE^2 - m^2), nextE = E0 - Eloss
+    double Elossb = 0.0;
+    if (m < nextE && pOut > 0.0) {
+      const double nEb = pOutb * nextE / pOut;
+      pb += nEb * p / E0;
+      Elossb = -nEb;
+    }
+    double epsb = 0.0;
+    if (el) {
+      // vQop = sq^2, sq = sqrt(q2b2) p^-2 (4 eps / conv)
+      const double sqb = 2.0 * sq * vQopb;
+      const double s2 = sqrt(q2b2), f = 4.0 / a.kLandauConv;
+      pb += sqb * (s2 * (-2.0 / (p * p * p)) * f * eps + (dq2b2 / (2.0 * s2)) * (1.0 / (p * p)) * f * eps);
+      epsb += sqb * s2 * (1.0 / (p * p)) * f;
+      // Eloss = eps running(p)
+      epsb += Elossb * running;
+      pb += Elossb * eps * drunP;
+    }
+    if (ms) {
+      // vPhi = (theta0 dn / rho)^2, vTheta = theta0^2
+      const double sigb = 2.0 * sigPhi * vPhib;
+      double theta0b = 2.0 * theta0 * vThetab + sigb * (dn / rho);
+      const double rb = sigb * theta0;  // adjoint of dn / rho
+      const double r3 = rho * rho * rho;
+      db[0] += rb * (d[0] / (dn * rho) - dn * d[0] / r3);
+      db[1] += rb * (d[1] / (dn * rho) - dn * d[1] / r3);
+      db[2] += rb * (d[2] / (dn * rho));
+      // theta0 = kH tt (1 + kHL log tt) / p, tt = sqrt(xOverX0 q2b2)
+      const double dth_dtt = a.kHighland / p * (1.0 + a.kHighlandLog * log(tt) + a.kHighlandLog);
+      const double ttb = theta0b * dth_dtt;
+      pb += theta0b * (-theta0 / p);
+      const double xOverX0 = pl.thicknessInX0 * pcor;
+      pb += ttb * (xOverX0 / (2.0 * tt)) * dq2b2;
+      pcorb += ttb * (q2b2 / (2.0 * tt)) * pl.thicknessInX0;
+    }
+    // eps = K thick(pcor) q2b2(p)
+    pb += epsb * depsP;
+    pcorb += epsb * depsC;
+    // pcor = 1 / |n . d|
+    const double cb = -pcorb / (c * c) * sgn;
+    const double nr[3] = {pl.r[2], pl.r[5], pl.r[8]};
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      db[k] += cb * nr[k];
+      nb[k] += cb * d[k];
+    }
+    return pb;
+  }
+};
 
 // ---------------------------------------------------------------------------
 // The reverse sweep of one track.
@@ -974,8 +1101,27 @@ struct TrackAdjoint {
       double cb[3] = {0, 0, 0}, Rb[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
       double dPre[3] = {0, 0, 0};  // adjoint of the direction at the plane (before the update)
       jacInitVJP(dEnd, Ab, Gb, dPre, Rb);
+      // material on this plane, recomputed from the filtered state
+      const double pF = fabs(1.0 / P.xf[4]);
+      double dF[3];
+      {
+        const double ct = cos(P.xf[3]), stt = sin(P.xf[3]);
+        const double v0 = cos(P.xf[2]) * stt, v1 = sin(P.xf[2]) * stt, v2 = ct;
+        const double vn = sqrt(v0 * v0 + v1 * v1 + v2 * v2);
+        dF[0] = v0 / vn;
+        dF[1] = v1 / vn;
+        dF[2] = v2 / vn;
+      }
+      MaterialFwd Mt;
+      Mt.run(a, pl, pF, dF);
+      // adjoint of the momentum after the plane from that of q/p
+      const double pAfterb = -qb * tf.q / (Mt.pOut * Mt.pOut);
+      double dbm[3] = {db[0], db[1], db[2]}, nbm[3] = {0, 0, 0};
+      const double pFb = Mt.vjp(a, pl, dF, pAfterb, Cb[2][2], Cb[3][3], Cb[4][4], dbm, nbm);
+#pragma unroll
+      for (int m = 0; m < 3; ++m) Rb[m][2] += nbm[m];
       double xfb[5] = {0, 0, 0, 0, 0};
-      freeFromBoundVJP(P, tf.q, pb, db, qb, xfb, cb, Rb);
+      freeFromBoundVJP(P, pb, dbm, pFb, xfb, cb, Rb);
       double posb[3], dbb[3], Abb[3][5], Gbb[3][3], k4b[3], qbb = 0.0, Cmb[5][5], cb2[3],
           Rb2[3][3];
       P.vjp(dEnd, posEnd, AEnd, k4, xfb, Cb, posb, dbb, Abb, Gbb, k4b, qbb, Cmb, cb2, Rb2);

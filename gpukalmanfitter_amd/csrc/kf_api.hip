@@ -16,6 +16,7 @@
 #include "../../include/kf_abi.h"
 #include "kf_adjoint.hpp"
 #include "kf_kernels.hpp"
+#include "kf_setup.hpp"
 
 struct kf_ctx {
   int device = 0;
@@ -90,19 +91,6 @@ int grow(kf_ctx *ctx, void **buf, size_t *cap, size_t need, hipStream_t s, const
   return KF_OK;
 }
 
-// Physics constants evaluated the way the reference's unit literals do
-// (long double literal times a double unit, rounded to double;
-// Interactions.ipp:19-25, Units.hpp:145-153) and its float literals.
-void physicsConstants(kfd::FitArgs &a) {
-  a.kMe = (double)(0.001L * 0.5109989461L);
-  a.kK = ((double)(0.001L * 0.307075L)) * 10.0 * 10.0;
-  a.kPlasma = (double)(1e-9L * 28.816L);
-  a.kHighland = (double)(0.001L * 13.6L);
-  const float c = 0.038f * 2;
-  a.kHighlandLog = (double)c;
-  const float conv = 2 * std::sqrt(2 * std::log(2.0f));
-  a.kLandauConv = (double)conv;
-}
 
 }  // namespace
 
@@ -184,25 +172,7 @@ int kf_set_geometry(kf_ctx *ctx, const kf_plane *planes, int32_t n_planes) {
   if (!planes || n_planes <= 0)
     return fail(ctx, KF_E_INVALID, "kf_set_geometry: need n_planes > 0 planes");
   std::vector<kfd::DevPlane> dp(n_planes);
-  for (int i = 0; i < n_planes; ++i) {
-    const kf_plane &p = planes[i];
-    kfd::DevPlane &d = dp[i];
-    std::memset(&d, 0, sizeof d);
-    for (int k = 0; k < 3; ++k) d.c[k] = p.center[k];
-    for (int k = 0; k < 9; ++k) d.r[k] = p.rot[k];
-    // MaterialSlab (MaterialSlab.ipp:15-20) and Material (Material.ipp:80-85,
-    // Material.hpp:102-104), only when the slab is valid (MaterialSlab.hpp:46-48)
-    const bool valid = p.thickness > 0.0 && 0.0f < p.ar;
-    d.has_material = valid ? 1 : 0;
-    if (valid) {
-      const double eps = 2 * DBL_EPSILON;
-      d.thickness = p.thickness;
-      d.thicknessInX0 = (eps < p.x0) ? (p.thickness / p.x0) : 0.0;
-      d.meanExcitation = (16.0 * 1e-9) * std::pow(p.z, (double)0.9f);
-      d.molarElectronDensity = p.z * p.molar_rho;
-    }
-    d.geo_id = p.geo_id;
-  }
+  for (int i = 0; i < n_planes; ++i) dp[i] = kfd::devPlane(planes[i]);
   KF_HIP(ctx, hipSetDevice(ctx->device));
   if (ctx->planes && ctx->n_planes != n_planes) {
     KF_HIP(ctx, hipFree(ctx->planes));
@@ -319,7 +289,7 @@ int setupArgs(kf_ctx *ctx, const kf_tracks *tr, const kf_options *opt, const cha
   a.max_step_size = opt->max_step_size;
   a.target_tolerance = opt->target_tolerance;
   a.overstep_limit = opt->overstep_limit;
-  physicsConstants(a);
+  kfd::physicsConstants(a);
   return KF_OK;
 }
 
@@ -523,10 +493,6 @@ int kf_chi2_gradient(kf_ctx *ctx, const kf_tracks *tr, const kf_options *opt, do
   kfd::FitArgs a;
   int rc = setupArgs(ctx, tr, opt, "kf_chi2_gradient", a);
   if (rc != KF_OK) return rc;
-  if ((opt->multiple_scattering || opt->energy_loss) && ctx->has_material)
-    return fail(ctx, KF_E_INVALID,
-                "kf_chi2_gradient: material effects are not differentiated; disable "
-                "multiple_scattering and energy_loss or use planes without material");
   a.smoothing = 0;  // the objective is the forward filter's chi2
   KF_HIP(ctx, hipSetDevice(ctx->device));
   hipStream_t s = (hipStream_t)stream;

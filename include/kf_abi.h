@@ -214,8 +214,8 @@ int kf_fit(kf_ctx *ctx, const kf_tracks *tracks, const kf_options *opt,
  *                     with their own alignment parameterisation;
  *   track_flags       device [n_tracks] or NULL: 0, or KF_ADJ_* bits of a
  *                     track left out of the gradient.
- * Material effects are not differentiated: with multiple_scattering or
- * energy_loss on and a plane with material the call fails.  No reference
+ * Material effects (energy loss, scattering and straggling variances) are
+ * differentiated with the rest.  No reference
  * counterpart (this build's addition; the objective is defined in
  * results.chi2). */
 #define KF_ADJ_SKIPPED 0x1u   /* no measured surface */

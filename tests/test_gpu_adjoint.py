@@ -2,7 +2,8 @@
 
 Objective X = summed predicted-residual chi2 of the FP64 forward filter over
 the batch; gradient with respect to 6 alignment parameters per plane (local
-shifts and rotations, gpukalmanfitter_amd/alignment.py) and the field scale.
+shifts and rotations, gpukalmanfitter_amd/alignment.py) and the field scale,
+with and without material effects.
 Checks: the objective equals the FP64 fit's chi2 sum; every gradient
 component matches central finite differences of the FP64 fit to 1e-4, with
 steps of 1e-6 mm for shifts, 1e-6 (relative) for the field scale and 1e-8 rad
@@ -194,15 +195,20 @@ def test_adjoint_behind_torch_autograd(torch_cuda):
     assert abs(float(scale.grad) - fds) <= TOL * abs(fds)
 
 
-def test_adjoint_refuses_material(torch_cuda):
-    from gpukalmanfitter_amd.fitter import (DeviceBatch, FitError, KalmanFitter,
-                                            KalmanFitterOptions)
-    b = G.generate(5, n_tracks=100)
+def test_adjoint_material_config5(torch_cuda):
+    """Config 5 sample (300 um silicon on every plane, multiple scattering and
+    energy loss on, grid field): the material update is differentiated too."""
+    from gpukalmanfitter_amd.fitter import DeviceBatch, KalmanFitter, KalmanFitterOptions
+    b = G.generate(5, n_tracks=20000)
     kf = KalmanFitter(b.planes, b.field)
     dev = DeviceBatch(b)
-    with pytest.raises(FitError, match="material"):
-        kf.chi2_gradient(dev, KalmanFitterOptions(**configs.WORKLOAD_FIT_OPTIONS))
-    # without material effects the same batch is differentiated
-    _, _, _, flags, _ = kf.chi2_gradient(dev, KalmanFitterOptions(**W))
-    assert (flags == 0).all()
+    opts = KalmanFitterOptions(**configs.WORKLOAD_FIT_OPTIONS)
+    grad, gscale, chi2, flags, raw = kf.chi2_gradient(dev, opts)
+    assert (flags == 0).all(), np.unique(flags, return_counts=True)
+    x = chi2_of(kf, dev, opts)
+    assert abs(chi2 - x) <= 1e-12 * abs(x), (chi2, x)
+    comps = [(p, j) for p in (0, 9, 25, 39) for j in range(6)]
+    worst, ds = fd_check(kf, dev, opts, b.planes, 1.0, grad, gscale, comps)
     kf.close()
+    # grid-field cell faces, as in the config-3 check
+    assert worst <= 1e-3 and ds <= TOL, (worst, ds)
