@@ -42,9 +42,14 @@ FLOPS32_BWD_PER_TRACK = 784.0
 FP64_PEAK_TFLOPS = 78.6
 FP32_PEAK_TFLOPS = 157.3
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per launch from rocprofv3 FETCH_SIZE + WRITE_SIZE (scaled by the
-# copy-kernel calibration), per precision; None until profiled.
-TRAFFIC_BYTES_PER_LAUNCH = {}
+# HBM bytes per launch (fetched + written) of the two fit kernels on this
+# workload, from rocprofv3 FETCH_SIZE / WRITE_SIZE scaled by the copy-kernel
+# calibration (profiles/kf_fit_config2_1M_tracks.md).
+TRAFFIC_TRACKS = 1048576
+TRAFFIC_BYTES_PER_LAUNCH = {
+    ("fp64", "forward"): 4.857e9 + 5.411e9, ("fp64", "backward"): 5.565e9 + 0.231e9,
+    ("mixed", "forward"): 4.640e9 + 3.355e9, ("mixed", "backward"): 3.230e9 + 0.231e9,
+}
 PARAM_NAMES = ("loc0", "loc1", "phi", "theta", "qop", "t")
 
 
@@ -125,16 +130,20 @@ def roofline(prec, n, phase_ms):
            "backward": FLOPS32_BWD_PER_TRACK if mixed else 0.0}
     ms = {"forward": phase_ms[0], "backward": phase_ms[1]}
     k = max(ms, key=ms.get)
+    traffic = TRAFFIC_BYTES_PER_LAUNCH.get((prec, k))
+    if traffic is not None:
+        traffic *= n / TRAFFIC_TRACKS
     f, g = flops[k], f32[k]
     t_roof_per_track = g / (FP32_PEAK_TFLOPS * 1e12) + (f - g) / (FP64_PEAK_TFLOPS * 1e12)
     peak = f / t_roof_per_track / 1e12
     achieved = f * n / (ms[k] * 1e-3) / 1e12
     other = "backward" if k == "forward" else "forward"
     return {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
-            "frac": achieved / peak, "traffic": TRAFFIC_BYTES_PER_LAUNCH.get((prec, k)),
+            "frac": achieved / peak, "traffic": traffic,
             "kernel": "kf_fit_kernel<%s, %s>" % ("float" if mixed else "double",
                                                   "true" if k == "forward" else "false"),
             "kernel_ms": ms[k], "flops_per_track": f, "fp32_flops_per_track": g,
+            "hbm_tb_per_s": (traffic or 0.0) / (ms[k] * 1e-3) / 1e12,
             "compute": "vector ALU, not MFMA: peak = F / (F32/157.3 + F64/78.6 TFLOP/s), the "
                        "MI355X fp32/fp64 vector peaks weighted by this launch's flop mix",
             "other_launch": {"kernel_ms": ms[other], "flops_per_track": flops[other],

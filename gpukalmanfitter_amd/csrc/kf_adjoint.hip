@@ -17,19 +17,25 @@ __global__ void __launch_bounds__(KF_BLOCK) kf_adjoint_kernel(FitArgs args, AdjA
   ta.run();
 }
 
-// Block p < n_planes sums the 12 plane adjoints of plane p over all tracks;
-// block n_planes the field-scale adjoint; block n_planes + 1 the chi2.  Each
-// thread sums a fixed stride of tracks, then a fixed-order tree: the result
-// does not depend on scheduling.
+// Two-stage sum over tracks.  Stage 1: block (p, b) sums slice b of the
+// tracks for quantity p (p < n_planes: the 12 adjoints of plane p; p =
+// n_planes: the field-scale adjoint; p = n_planes + 1: the chi2) into
+// partial[p][b]; each thread sums a fixed stride, then a fixed-order tree.
+// Stage 2 adds the partials of each p in slice order.  The result does not
+// depend on scheduling.
 constexpr int kRedThreads = 256;
+constexpr int kRedSlices = KF_ADJ_RED_SLICES;
 __global__ void __launch_bounds__(kRedThreads) kf_adjoint_reduce(FitArgs args, AdjArgs adj,
-                                                                 double *sums) {
+                                                                 double *partial) {
   __shared__ double red[kRedThreads][kPG_N];
   const int p = blockIdx.x;
+  const int b = blockIdx.y;
+  const int64_t per = (args.n + kRedSlices - 1) / kRedSlices;
+  const int64_t lo = (int64_t)b * per, hi = (lo + per < args.n) ? lo + per : args.n;
   double acc[kPG_N];
 #pragma unroll
   for (int f = 0; f < kPG_N; ++f) acc[f] = 0.0;
-  for (int64_t i = threadIdx.x; i < args.n; i += kRedThreads) {
+  for (int64_t i = lo + threadIdx.x; i < hi; i += kRedThreads) {
     if (p < args.n_planes) {
       const int q = p - args.seq_first[i];
       if (q >= 0 && q < args.seq_count[i] && q < adj.max_seq) {
@@ -52,7 +58,15 @@ __global__ void __launch_bounds__(kRedThreads) kf_adjoint_reduce(FitArgs args, A
     }
     __syncthreads();
   }
-  if (threadIdx.x < kPG_N) sums[(size_t)p * kPG_N + threadIdx.x] = red[0][threadIdx.x];
+  if (threadIdx.x < kPG_N)
+    partial[((size_t)p * kRedSlices + b) * kPG_N + threadIdx.x] = red[0][threadIdx.x];
+}
+
+__global__ void __launch_bounds__(kPG_N) kf_adjoint_reduce2(const double *partial, double *sums) {
+  const int p = blockIdx.x;
+  double s = 0.0;
+  for (int b = 0; b < kRedSlices; ++b) s += partial[((size_t)p * kRedSlices + b) * kPG_N + threadIdx.x];
+  sums[(size_t)p * kPG_N + threadIdx.x] = s;
 }
 
 // dX/d(du, dv, dw) = R^T dX/dc; dX/d(a, b, c) = <dX/dR, R G> with G the
@@ -96,8 +110,11 @@ extern "C" hipError_t kfd_launch_adjoint(const kfd::FitArgs *args, const kfd::Ad
   if (blocks > 0)
     hipLaunchKernelGGL(kfd::kf_adjoint_kernel, dim3((unsigned)blocks), dim3(KF_BLOCK), 0, stream,
                        *args, *adj);
-  hipLaunchKernelGGL(kfd::kf_adjoint_reduce, dim3((unsigned)(args->n_planes + 2)),
-                     dim3(kfd::kRedThreads), 0, stream, *args, *adj, sums);
+  double *partial = sums + (size_t)(args->n_planes + 2) * kfd::kPG_N;
+  hipLaunchKernelGGL(kfd::kf_adjoint_reduce, dim3((unsigned)(args->n_planes + 2), kfd::kRedSlices),
+                     dim3(kfd::kRedThreads), 0, stream, *args, *adj, partial);
+  hipLaunchKernelGGL(kfd::kf_adjoint_reduce2, dim3((unsigned)(args->n_planes + 2)),
+                     dim3(kfd::kPG_N), 0, stream, (const double *)partial, sums);
   hipLaunchKernelGGL(kfd::kf_adjoint_finish, dim3((unsigned)((args->n_planes + 63) / 64)),
                      dim3(64), 0, stream, *args, (const double *)sums, grad_planes, grad_scale,
                      chi2);

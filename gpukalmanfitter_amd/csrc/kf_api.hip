@@ -509,7 +509,8 @@ int kf_chi2_gradient(kf_ctx *ctx, const kf_tracks *tr, const kf_options *opt, do
   const size_t b_pg = nplanes * kfd::kPG_N * n * sizeof(double);
   const size_t b_scale = n * sizeof(double);
   const size_t b_flags = n * sizeof(uint32_t);
-  const size_t b_sums = (nplanes + 2) * kfd::kPG_N * sizeof(double);
+  // sums [n_planes + 2][12] followed by the reduction's partials
+  const size_t b_sums = (nplanes + 2) * kfd::kPG_N * (1 + KF_ADJ_RED_SLICES) * sizeof(double);
   auto al = [](size_t x) { return (x + 255) / 256 * 256; };
   rc = grow(ctx, &ctx->adj, &ctx->adj_bytes,
             al(b_out) + al(b_stack) + al(b_pg) + al(b_scale) + al(b_flags) + al(b_sums), s,

@@ -27,11 +27,12 @@ struct ChunkBases {
 
 // Fits the track at position pos (pass FWD of the two kernels).
 template <typename S, bool FWD>
-KFD_INLINE void fitPosition(const FitArgs &a, int64_t pos) {
+KFD_INLINE void fitPosition(const FitArgs &a, int64_t pos, const DevPlane *planes) {
   const ChunkBases<S> cb(a, pos);
   if (pos >= a.n) return;
   const int64_t i = a.order ? (int64_t)a.order[pos] : pos;
   TrackFit<S, FWD> tf(a, i, cb.d, cb.s, cb.f, cb.c, (uint32_t)(pos & 63));
+  tf.pls = planes;
   if constexpr (FWD)
     tf.runForward();
   else
@@ -41,8 +42,21 @@ KFD_INLINE void fitPosition(const FitArgs &a, int64_t pos) {
 // One thread per position, or (persistent) resident waves that take
 // chunks_per_grab 64-position chunks at a time from a work-queue counter
 // until every chunk is taken; the grab is one vector atomic by lane 0.
-template <typename S, bool FWD>
+//
+// LDS = true: the block first copies the plane table (at most KF_LDS_PLANES
+// planes) to LDS and reads planes there.
+template <typename S, bool FWD, bool LDS>
 __global__ void __launch_bounds__(KF_BLOCK) kf_fit_kernel(FitArgs args) {
+  const DevPlane *planes = args.planes;
+  if constexpr (LDS) {
+    __shared__ DevPlane lds_planes[KF_LDS_PLANES];
+    const double *src = (const double *)args.planes;
+    double *dst = (double *)lds_planes;
+    const int words = args.n_planes * (int)(sizeof(DevPlane) / sizeof(double));
+    for (int w = threadIdx.x; w < words; w += KF_BLOCK) dst[w] = src[w];
+    __syncthreads();
+    planes = lds_planes;
+  }
   const uint32_t lane = threadIdx.x & 63u;
   const uint32_t nchunks = (uint32_t)((args.n + 63) >> 6);
   const uint32_t grab = (uint32_t)args.chunks_per_grab;
@@ -59,7 +73,7 @@ __global__ void __launch_bounds__(KF_BLOCK) kf_fit_kernel(FitArgs args) {
       c = c0;
       c1 = (c0 + grab < nchunks) ? c0 + grab : nchunks;
     }
-    for (; c < c1; ++c) fitPosition<S, FWD>(args, (int64_t)c * 64 + lane);
+    for (; c < c1; ++c) fitPosition<S, FWD>(args, (int64_t)c * 64 + lane, planes);
     if (!args.persistent) break;
   }
 }
@@ -160,18 +174,28 @@ extern "C" hipError_t kfd_launch_fit(const kfd::FitArgs *args, int precision, hi
   }
   const int64_t blocks = pers ? launch->persistent_blocks : (args->n + KF_BLOCK - 1) / KF_BLOCK;
   const dim3 grid((unsigned)blocks), block(KF_BLOCK);
-  if (precision == KF_PRECISION_FP64)
-    hipLaunchKernelGGL((kfd::kf_fit_kernel<double, true>), grid, block, 0, stream, *args);
+  const bool lds = args->n_planes <= KF_LDS_PLANES;
+  const bool fp64 = precision == KF_PRECISION_FP64;
+#define KFD_LAUNCH(S_, FWD_)                                                                  \
+  do {                                                                                        \
+    if (lds)                                                                                  \
+      hipLaunchKernelGGL((kfd::kf_fit_kernel<S_, FWD_, true>), grid, block, 0, stream, *args);  \
+    else                                                                                      \
+      hipLaunchKernelGGL((kfd::kf_fit_kernel<S_, FWD_, false>), grid, block, 0, stream, *args); \
+  } while (0)
+  if (fp64)
+    KFD_LAUNCH(double, true);
   else
-    hipLaunchKernelGGL((kfd::kf_fit_kernel<float, true>), grid, block, 0, stream, *args);
+    KFD_LAUNCH(float, true);
   if (between) {
     const hipError_t e = hipEventRecord(between, stream);
     if (e != hipSuccess) return e;
   }
-  if (precision == KF_PRECISION_FP64)
-    hipLaunchKernelGGL((kfd::kf_fit_kernel<double, false>), grid, block, 0, stream, *args);
+  if (fp64)
+    KFD_LAUNCH(double, false);
   else
-    hipLaunchKernelGGL((kfd::kf_fit_kernel<float, false>), grid, block, 0, stream, *args);
+    KFD_LAUNCH(float, false);
+#undef KFD_LAUNCH
   if (track_states_out) {
     const int64_t eb = (args->n + 255) / 256;
     hipLaunchKernelGGL(kfd::kf_export_states, dim3((unsigned)eb), dim3(256), 0, stream, *args,

@@ -8,6 +8,10 @@
 #include "kf_device.hpp"
 
 #define KF_BLOCK 128  // multiple of 64: a wave covers one 64-track store chunk
+// Plane tables up to this size are copied to LDS by every fit block (plane
+// reads then wait on LDS only, not behind the store writes' vector-memory
+// count); larger tables are read from global memory.
+#define KF_LDS_PLANES 256
 
 // per-state store fields (see FitArgs)
 #define KF_SD_XP 0   // predicted parameters (6)
@@ -100,6 +104,8 @@ struct FitArgs {
 // Step-stack depth per track of the chi2 adjoint: a segment between two
 // measured surfaces may take this many RK steps.
 #define KF_ADJ_MAX_SEGMENT_STEPS 48
+// Track slices of the adjoint's first reduction stage.
+#define KF_ADJ_RED_SLICES 64
 
 namespace kfd {
 struct AdjArgs;

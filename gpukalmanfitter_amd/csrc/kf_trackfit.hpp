@@ -71,11 +71,14 @@ struct TrackFit {
   S *tss;
   double *tsf;
   double *tsc;
+  // the plane table: a copy in LDS (set by the kernel) or a.planes
+  const DevPlane *pls;
   uint32_t lane;
 
   KFD_INLINE TrackFit(const FitArgs &args, int64_t idx, double *d_base, S *s_base, double *f_base,
                       double *c_base, uint32_t ln)
-      : a(args), i(idx), tsd(d_base), tss(s_base), tsf(f_base), tsc(c_base), lane(ln) {}
+      : a(args), i(idx), tsd(d_base), tss(s_base), tsf(f_base), tsc(c_base), pls(args.planes),
+        lane(ln) {}
 
   // ------------------------------------------------------------ storage ----
   // Element f of state k: base[(k * F + f) * 64 + lane] -- a 32-bit lane
@@ -627,7 +630,7 @@ struct TrackFit {
   }
 
   // ----------------------------------------------------------- navigation ----
-  KFD_INLINE const DevPlane &plane(int idx) const { return a.planes[idx]; }
+  KFD_INLINE const DevPlane &plane(int idx) const { return pls[idx]; }
 
   // updateSingleSurfaceStatus on a plane, SteppingHelper.hpp:31-65 (+
   // PlanarHelper.hpp:28-49); returns 0 unreachable, 1 reachable, 2 on surface
@@ -805,13 +808,21 @@ struct TrackFit {
   // by construction.  The 2x2 inversion and the update are double in both.
   KFD_INLINE bool filter(int sp) {
     const DevPlane &pl = plane(sp);
+    // the track's hit on this plane: first match in hit order, four
+    // independent geo-id loads per round trip
     int64_t hit = -1;
-    for (int64_t h = h0; h < h1; ++h)
-      if (a.hit_geo_id[h] == pl.geo_id) {
-        hit = h;
-        break;
-      }
+    for (int64_t h = h0; h < h1 && hit < 0; h += 4) {
+      int32_t g[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) g[q] = (h + q < h1) ? a.hit_geo_id[h + q] : 0;
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+        if (hit < 0 && h + q < h1 && g[q] == pl.geo_id) hit = h + q;
+    }
     if (hit < 0) return true;  // hole
+    const double m0 = a.hit_meas[hit], m1 = a.hit_meas[a.n_hits + hit];
+    const double V00 = a.hit_cov[hit], V01 = a.hit_cov[a.n_hits + hit];
+    const double V11 = a.hit_cov[2 * a.n_hits + hit];
     if (measured >= a.max_states) return false;
     const int k = measured;
     if (k == 0) firstPlane = sp;
@@ -836,9 +847,6 @@ struct TrackFit {
       *recF(k, KF_TS_GEO_ID) = (double)pl.geo_id;
     }
     // gain-matrix update
-    const double m0 = a.hit_meas[hit], m1 = a.hit_meas[a.n_hits + hit];
-    const double V00 = a.hit_cov[hit], V01 = a.hit_cov[a.n_hits + hit];
-    const double V11 = a.hit_cov[2 * a.n_hits + hit];
     const double S00 = cov[sidx(0, 0)] + V00, S01 = cov[sidx(0, 1)] + V01;
     const double S10 = cov[sidx(1, 0)] + V01, S11 = cov[sidx(1, 1)] + V11;
     const double det = S00 * S11 - S01 * S10;
