@@ -10,16 +10,20 @@ whole batch with inputs already resident in HBM; the timed region is exactly
 given by --precision (first one is the headline `value`).
 
 A fit is two launches on one stream: the forward filter (start, steps,
-gain-matrix update on every surface) and the smoother plus back-propagation
-to the perigee.  The roofline object prices the dominant one of the headline
-precision against MI355X's vector peaks, fp64 78.6 TFLOP/s and fp32
-157.3 TFLOP/s, with T_roof = F32 / P32 + F64 / P64 per track: F is the
-algorithmic flop count of the reference formulation (structural zeros and
-identities removed) of that launch's share of the fit, F32 the part the mixed
-mode runs in float (0 in fp64 mode), both measured by the counting build of
-the CPU oracle on this workload (constants below).  Launch durations are the
-mean over the timed steps of HIP events the library records on the launch
-stream before, between and after the two launches.
+gain-matrix update on every surface, per-surface store writes) and the
+smoother plus back-propagation to the perigee (store reads).  The roofline
+object is for the longer launch of the headline precision.  Both launches sit
+below the ridge point of the vector ALU (2.2-5.2 algorithmic flops per HBM
+byte against 10-14), so the bound is HBM: achieved = algorithmic bytes of the
+launch (inputs, outputs, smoother store, hand-over; ALG_BYTES_PER_TRACK) per
+second, peak 8 TB/s, traffic = the FETCH_SIZE + WRITE_SIZE bytes rocprofv3
+measured for the launch.  The compute side is reported beside it: flops are
+the algorithmic count of the reference formulation (structural zeros and
+identities removed) measured by the counting build of the CPU oracle, priced
+at F / (F32 / 157.3 + F64 / 78.6 TFLOP/s), the MI355X vector peaks weighted by
+the launch's float share.  Launch durations are the mean over the timed steps
+of HIP events the library records on the launch stream before, between and
+after the two launches.
 """
 import argparse
 import json
@@ -41,6 +45,17 @@ FLOPS32_PER_TRACK = 8109.0
 FLOPS32_BWD_PER_TRACK = 784.0
 FP64_PEAK_TFLOPS = 78.6
 FP32_PEAK_TFLOPS = 157.3
+# Algorithmic HBM bytes per config-2 track and launch (10 hits, 10 measured
+# states; tests/test_bench_constants.py re-derives them from the layouts):
+#   forward  = inputs 696 (start parameters 48 + covariance 168 + perigee 24
+#              + sequence 8 + hit offset 8 + 10 hits x 44) + chi2/ndf/n_measured
+#              16 + hand-over 264 (6 int32 + 30 double) + store writes
+#              10 x (12 x 8 + 49 x sizeof(S));
+#   backward = store reads 10 x (12 x 8 + 49 x sizeof(S)) + hand-over 264 +
+#              perigee, sequence and hit offsets 48 + outputs 220 (params 48,
+#              cov 168, status 4).
+ALG_BYTES_PER_TRACK = {("fp64", "forward"): 5856, ("fp64", "backward"): 5412,
+                       ("mixed", "forward"): 3896, ("mixed", "backward"): 3452}
 HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch (fetched + written) of the two fit kernels on this
 # workload, from rocprofv3 FETCH_SIZE / WRITE_SIZE scaled by the copy-kernel
@@ -122,35 +137,42 @@ def cpu_baseline(batch, opts_c, threads, sample, gpu_results):
     return base, errors
 
 
-def roofline(prec, n, phase_ms):
-    """Roofline of the dominant launch (see the module docstring)."""
+def launch_figures(prec, phase, n, ms):
+    """Algorithmic bytes and flops of one launch of `phase` and what they make."""
     mixed = prec == "mixed"
-    flops = {"forward": FLOPS_PER_TRACK - FLOPS_BWD_PER_TRACK, "backward": FLOPS_BWD_PER_TRACK}
-    f32 = {"forward": (FLOPS32_PER_TRACK - FLOPS32_BWD_PER_TRACK) if mixed else 0.0,
-           "backward": FLOPS32_BWD_PER_TRACK if mixed else 0.0}
-    ms = {"forward": phase_ms[0], "backward": phase_ms[1]}
-    k = max(ms, key=ms.get)
-    traffic = TRAFFIC_BYTES_PER_LAUNCH.get((prec, k))
+    f = FLOPS_PER_TRACK - FLOPS_BWD_PER_TRACK if phase == "forward" else FLOPS_BWD_PER_TRACK
+    g = 0.0
+    if mixed:
+        g = FLOPS32_PER_TRACK - FLOPS32_BWD_PER_TRACK if phase == "forward" else FLOPS32_BWD_PER_TRACK
+    peak_flops = f / (g / FP32_PEAK_TFLOPS + (f - g) / FP64_PEAK_TFLOPS)  # TFLOP/s
+    b = ALG_BYTES_PER_TRACK[(prec, phase)]
+    t = ms * 1e-3
+    traffic = TRAFFIC_BYTES_PER_LAUNCH.get((prec, phase))
     if traffic is not None:
         traffic *= n / TRAFFIC_TRACKS
-    f, g = flops[k], f32[k]
-    t_roof_per_track = g / (FP32_PEAK_TFLOPS * 1e12) + (f - g) / (FP64_PEAK_TFLOPS * 1e12)
-    peak = f / t_roof_per_track / 1e12
-    achieved = f * n / (ms[k] * 1e-3) / 1e12
+    return {"kernel": "kf_fit_kernel<%s, %s>" % ("float" if mixed else "double",
+                                                "true" if phase == "forward" else "false"),
+            "kernel_ms": ms, "bytes_per_track": b, "flops_per_track": f,
+            "fp32_flops_per_track": g, "intensity_flop_per_byte": f / b,
+            "hbm_gb_per_s": b * n / t / 1e9, "hbm_frac": b * n / t / 1e9 / HBM_PEAK_GBS,
+            "tflop_per_s": f * n / t / 1e12, "vector_peak_tflop_per_s": peak_flops,
+            "vector_frac": f * n / t / 1e12 / peak_flops, "traffic": traffic}
+
+
+def roofline(prec, n, phase_ms):
+    """Roofline of the dominant launch (see the module docstring)."""
+    ms = {"forward": phase_ms[0], "backward": phase_ms[1]}
+    k = max(ms, key=ms.get)
     other = "backward" if k == "forward" else "forward"
-    return {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
-            "frac": achieved / peak, "traffic": traffic,
-            "kernel": "kf_fit_kernel<%s, %s>" % ("float" if mixed else "double",
-                                                  "true" if k == "forward" else "false"),
-            "kernel_ms": ms[k], "flops_per_track": f, "fp32_flops_per_track": g,
-            "hbm_tb_per_s": (traffic or 0.0) / (ms[k] * 1e-3) / 1e12,
-            "compute": "vector ALU, not MFMA: peak = F / (F32/157.3 + F64/78.6 TFLOP/s), the "
-                       "MI355X fp32/fp64 vector peaks weighted by this launch's flop mix",
-            "other_launch": {"kernel_ms": ms[other], "flops_per_track": flops[other],
-                             "fp32_flops_per_track": f32[other],
-                             "frac": flops[other] * n / (ms[other] * 1e-3) / 1e12 /
-                             (flops[other] / (f32[other] / FP32_PEAK_TFLOPS +
-                                              (flops[other] - f32[other]) / FP64_PEAK_TFLOPS))}}
+    dom = launch_figures(prec, k, n, ms[k])
+    return {"bound": "hbm", "achieved": dom["hbm_gb_per_s"], "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": dom["hbm_frac"], "traffic": dom["traffic"],
+            "kernel": dom["kernel"], "kernel_ms": ms[k],
+            "why_hbm": "arithmetic intensity %.2f flop/B is below the ridge point of the vector "
+                       "ALU (%.1f TFLOP/s / 8 TB/s = %.1f flop/B)" % (
+                           dom["intensity_flop_per_byte"], dom["vector_peak_tflop_per_s"],
+                           dom["vector_peak_tflop_per_s"] / 8.0),
+            "dominant_launch": dom, "other_launch": launch_figures(prec, other, n, ms[other])}
 
 
 def adjoint_leg(steps, warmup):

@@ -25,3 +25,24 @@ def test_flops_per_track_constants(built):
                            ("FLOPS32_BWD_PER_TRACK", o.flops32_bwd)):
         c = const(name)
         assert abs(measured.mean() - c) / c < 0.01, (name, measured.mean())
+
+
+def test_algorithmic_bytes_per_track_constants():
+    """bench.ALG_BYTES_PER_TRACK from the store layout of the kernels
+    (include/kf_abi.h, gpukalmanfitter_amd/csrc/kf_kernels.hpp) and the
+    config-2 track shape (10 hits, 10 measured states)."""
+    src = open(os.path.join(ROOT, "bench.py")).read()
+    hdr = open(os.path.join(ROOT, "gpukalmanfitter_amd", "csrc", "kf_kernels.hpp")).read()
+    sd = int(re.search(r"#define KF_SD_N (\d+)", hdr).group(1))
+    ss = int(re.search(r"#define KF_SS_N (\d+)", hdr).group(1))
+    import ast
+    table = ast.literal_eval(re.search(r"ALG_BYTES_PER_TRACK = (\{.*?\})", src, re.S).group(1))
+    b = G.generate(2, n_tracks=1000)
+    assert (np.diff(b.hit_offset) == 10).all() and (b.seq_count == 10).all()
+    hits = 10 * (4 + 2 * 8 + 3 * 8)
+    inputs = 6 * 8 + 21 * 8 + 3 * 8 + 2 * 4 + 8 + hits
+    carry = 6 * 4 + 30 * 8
+    for prec, ssize in (("fp64", 8), ("mixed", 4)):
+        store = 10 * (sd * 8 + ss * ssize)
+        assert table[(prec, "forward")] == inputs + 16 + carry + store
+        assert table[(prec, "backward")] == store + carry + 48 + (6 * 8 + 21 * 8 + 4)
