@@ -83,6 +83,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-ragged", action="store_true",
                     help="skip the ragged-batch leg (config 3, 2,000,000 tracks)")
+    ap.add_argument("--no-material", action="store_true",
+                    help="skip the 4,194,304-track material workload leg")
     ap.add_argument("--no-adjoint", action="store_true",
                     help="skip the chi2-adjoint leg (config 4, 262,144 tracks)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -103,12 +105,14 @@ def copy_calibration(mib):
     print(json.dumps({"copy_calibration_bytes": n * 4}), flush=True)
 
 
-def cpu_baseline(batch, opts_c, threads, sample, gpu_results):
+def cpu_baseline(batch, opts_c, threads, sample, gpu_results, others=None):
     """The CPU oracle (FP64 restatement of the reference fit, OpenMP over
     tracks as Run/CPU/KalmanFitterCPUTest.cpp) timed on a bounded sample.
     Its results on that sample also give the per-parameter error of each GPU
     precision mode (|x_gpu - x_cpu| / max(|x_cpu|, sigma_cpu), max and 99.9 %
-    quantile over the fitted tracks)."""
+    quantile over the fitted tracks); `others` maps a name to (batch sample,
+    GPU results on it) of another workload, checked the same way (the third
+    value returned)."""
     import numpy as np
     import oracle
     from gpukalmanfitter_amd import abi
@@ -123,6 +127,20 @@ def cpu_baseline(batch, opts_c, threads, sample, gpu_results):
                       f"(oracle/kfo_fitter.hpp), OpenMP over tracks, {threads} threads, "
                       f"fit loop timed alone ({sec:.2f} s)"}
     ores, _, _, _ = oracle.fit(sub, opts_c, n_threads=threads)
+    other_errors = {}
+    for name, (ob, ores_gpu) in (others or {}).items():
+        o, _, _, _ = oracle.fit(ob, opts_c, n_threads=threads)
+        other_errors[name] = dict(param_errors(ores_gpu, o, ob.n_tracks),
+                                  sample=f"first {ob.n_tracks} tracks")
+    return base, param_errors(gpu_results, ores, n), other_errors
+
+
+def param_errors(gpu_results, ores, n):
+    """Per-parameter deviation of the first n tracks of each GPU result from
+    the oracle's: |x_gpu - x_cpu| / max(|x_cpu|, sigma_cpu), max and 99.9 %
+    quantile over the tracks the oracle fitted."""
+    import numpy as np
+    from gpukalmanfitter_amd import abi
     ok = (ores.status & abi.KF_STATUS_OK) != 0
     sig = np.sqrt(np.abs(ores.cov[abi.DIAG_INDEX][:, ok]))
     scale = np.maximum(np.abs(ores.params[:, ok]), sig)
@@ -134,7 +152,7 @@ def cpu_baseline(batch, opts_c, threads, sample, gpu_results):
             "max": {nm: float(dev[k].max()) for k, nm in enumerate(PARAM_NAMES)},
             "q999": {nm: float(np.quantile(dev[k], 0.999)) for k, nm in enumerate(PARAM_NAMES)},
         }
-    return base, errors
+    return errors
 
 
 def launch_figures(prec, phase, n, ms):
@@ -247,6 +265,68 @@ def ragged_leg(steps, warmup, precision="mixed"):
             "default_schedule_tracks_per_s": b.n_tracks / default_s}
 
 
+def material_leg(steps, warmup, sample):
+    """The material workload: 4,194,304 tracks over 8-40 planes of 300 um
+    silicon (Highland scattering, Bethe energy loss) in the grid field.  The
+    FP32-mixed fit with the autotuned schedule, the FP64 fit with the same
+    schedule, and the chi2 adjoint with respect to 6 x 40 alignment parameters
+    and the field scale (material differentiated).  Returns the figures, the
+    first `sample` tracks and both fits' results on them (checked against
+    the oracle in the CPU leg).  The working set exceeds the library's
+    default memory budget, so every call runs in slices."""
+    import numpy as np
+    import torch
+    from gpukalmanfitter_amd import abi, configs
+    from gpukalmanfitter_amd import generate as G
+    from gpukalmanfitter_amd.fitter import DeviceBatch, KalmanFitter, KalmanFitterOptions
+    t0 = time.time()
+    b = G.generate(5)
+    print(f"[bench] generated {b.n_tracks} material tracks in {time.time() - t0:.1f}s",
+          file=sys.stderr, flush=True)
+    kf = KalmanFitter(b.planes, b.field)
+    dev = DeviceBatch(b)
+
+    def timed(fn):
+        for _ in range(warmup):
+            fn()
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(steps):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t) / steps
+
+    out = {"workload": "config5-material: 4194304 tracks, U{8..40} planes of 300 um silicon at "
+                       "25 mm pitch, p 0.5-20 GeV, Highland + Bethe, 101x101x201 FP32 grid field",
+           "memory_budget_bytes": 16 << 30}
+    res = {}
+    rates = {}
+    for prec in ("mixed", "fp64"):
+        opts = KalmanFitterOptions(precision=prec, **configs.WORKLOAD_FIT_OPTIONS)
+        if prec == "mixed":
+            best, _ = kf.autotune(dev, opts)
+            out["schedule"] = best
+        sec = timed(lambda: kf.fit_batch(dev, opts))
+        rates[prec] = b.n_tracks / sec
+        r = dev.results_numpy()
+        res[prec] = {k: (r[k][..., :sample].copy()) for k in ("params", "status")}
+        out["fitted_fraction_" + prec] = float(((r["status"] & abi.KF_STATUS_OK) != 0).mean())
+        print(f"[bench] material {prec}: {rates[prec]:.4g} tracks/s ({sec * 1e3:.2f} ms)",
+              file=sys.stderr, flush=True)
+    out["tracks_per_s"] = rates
+    out["mixed_over_fp64"] = rates["mixed"] / rates["fp64"]
+    opts = KalmanFitterOptions(**configs.WORKLOAD_FIT_OPTIONS)
+    sec = timed(lambda: kf.chi2_gradient(dev, opts, synchronize=False))
+    _, _, _, flags, _ = kf.chi2_gradient(dev, opts)
+    out["adjoint"] = {"tracks_per_s": b.n_tracks / sec, "ms_per_step": sec * 1e3,
+                      "parameters": 6 * len(b.planes) + 1,
+                      "tracks_left_out": int((flags != 0).sum()), "dtype": "f64"}
+    print(f"[bench] material adjoint: {b.n_tracks / sec:.4g} tracks/s ({sec * 1e3:.1f} ms)",
+          file=sys.stderr, flush=True)
+    kf.close()
+    return out, b.subset(np.arange(min(sample, b.n_tracks))), res
+
+
 def main():
     args = parse()
     if args.copy_calibration:
@@ -322,11 +402,19 @@ def main():
         adj = adjoint_leg(max(args.steps // 2, 3), 1)
         print(f"[bench] adjoint: {adj['tracks_per_s']:.4g} tracks/s, {adj['ms_per_step']:.3f} ms/step",
               file=sys.stderr, flush=True)
+    material = None
+    others = {}
+    if not args.no_material:
+        material, msub, mres = material_leg(max(args.steps // 4, 2), 1, 10000)
+        others["material"] = (msub, mres)
     cpu = None
     errors = None
     if not args.no_cpu:
-        cpu, errors = cpu_baseline(batch, abi.default_options(**configs.WORKLOAD_FIT_OPTIONS),
-                                   args.cpu_threads, args.cpu_sample, results)
+        cpu, errors, oerr = cpu_baseline(batch,
+                                         abi.default_options(**configs.WORKLOAD_FIT_OPTIONS),
+                                         args.cpu_threads, args.cpu_sample, results, others)
+        if material is not None:
+            material["error_vs_oracle"] = oerr["material"]
         print(f"[bench] cpu baseline {cpu['value']:.4g} tracks/s on {cpu['cores']} threads",
               file=sys.stderr, flush=True)
     out = {
@@ -354,6 +442,7 @@ def main():
         "error_vs_oracle": errors,
         "ragged": ragged,
         "adjoint": adj,
+        "material": material,
     }
     if "fp64" in rates and "mixed" in rates:
         out["mixed_over_fp64"] = rates["mixed"] / rates["fp64"]

@@ -8,21 +8,22 @@
 namespace kfd {
 
 __global__ void __launch_bounds__(KF_BLOCK) kf_adjoint_kernel(FitArgs args, AdjArgs adj) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const size_t chunk = __builtin_amdgcn_readfirstlane((uint32_t)(i >> 6));
+  const int64_t local = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t chunk = __builtin_amdgcn_readfirstlane((uint32_t)(local >> 6));
   double *ck = args.tsc + chunk * (size_t)args.max_states * KF_CK_N * 64u;
   double *stk = adj.stack + chunk * (size_t)adj.max_seg_steps * kSR_N * 64u;
-  if (i >= args.n) return;
-  TrackAdjoint ta(args, adj, i, ck, stk, (uint32_t)(i & 63));
+  if (local >= args.count) return;
+  TrackAdjoint ta(args, adj, args.first + local, local, ck, stk, (uint32_t)(local & 63));
   ta.run();
 }
 
-// Two-stage sum over tracks.  Stage 1: block (p, b) sums slice b of the
-// tracks for quantity p (p < n_planes: the 12 adjoints of plane p; p =
-// n_planes: the field-scale adjoint; p = n_planes + 1: the chi2) into
-// partial[p][b]; each thread sums a fixed stride, then a fixed-order tree.
-// Stage 2 adds the partials of each p in slice order.  The result does not
-// depend on scheduling.
+// Two-stage sum over the tracks of a launch.  Stage 1: block (p, b) sums
+// slice b of the positions for quantity p (p < n_planes: the 12 adjoints of
+// plane p; p = n_planes: the field-scale adjoint; p = n_planes + 1: the chi2)
+// into partial[p][b]; each thread sums a fixed stride, then a fixed-order
+// tree.  Stage 2 adds the partials of each p in slice order, onto the sums of
+// the previous launches unless this is the first.  The result does not depend
+// on scheduling.
 constexpr int kRedThreads = 256;
 constexpr int kRedSlices = KF_ADJ_RED_SLICES;
 __global__ void __launch_bounds__(kRedThreads) kf_adjoint_reduce(FitArgs args, AdjArgs adj,
@@ -30,20 +31,22 @@ __global__ void __launch_bounds__(kRedThreads) kf_adjoint_reduce(FitArgs args, A
   __shared__ double red[kRedThreads][kPG_N];
   const int p = blockIdx.x;
   const int b = blockIdx.y;
-  const int64_t per = (args.n + kRedSlices - 1) / kRedSlices;
-  const int64_t lo = (int64_t)b * per, hi = (lo + per < args.n) ? lo + per : args.n;
+  const int64_t per = (args.count + kRedSlices - 1) / kRedSlices;
+  const int64_t lo = (int64_t)b * per, hi = (lo + per < args.count) ? lo + per : args.count;
   double acc[kPG_N];
 #pragma unroll
   for (int f = 0; f < kPG_N; ++f) acc[f] = 0.0;
-  for (int64_t i = lo + threadIdx.x; i < hi; i += kRedThreads) {
+  for (int64_t l = lo + threadIdx.x; l < hi; l += kRedThreads) {
+    const int64_t i = args.first + l;
     if (p < args.n_planes) {
       const int q = p - args.seq_first[i];
       if (q >= 0 && q < args.seq_count[i] && q < adj.max_seq) {
 #pragma unroll
-        for (int f = 0; f < kPG_N; ++f) acc[f] += adj.plane_adj[((size_t)q * kPG_N + f) * args.n + i];
+        for (int f = 0; f < kPG_N; ++f)
+          acc[f] += adj.plane_adj[((size_t)q * kPG_N + f) * args.count + l];
       }
     } else if (p == args.n_planes) {
-      acc[0] += adj.scale_adj[i];
+      acc[0] += adj.scale_adj[l];
     } else {
       acc[0] += args.out_chi2[i];
     }
@@ -62,11 +65,13 @@ __global__ void __launch_bounds__(kRedThreads) kf_adjoint_reduce(FitArgs args, A
     partial[((size_t)p * kRedSlices + b) * kPG_N + threadIdx.x] = red[0][threadIdx.x];
 }
 
-__global__ void __launch_bounds__(kPG_N) kf_adjoint_reduce2(const double *partial, double *sums) {
+__global__ void __launch_bounds__(kPG_N) kf_adjoint_reduce2(const double *partial, double *sums,
+                                                            int accumulate) {
   const int p = blockIdx.x;
   double s = 0.0;
   for (int b = 0; b < kRedSlices; ++b) s += partial[((size_t)p * kRedSlices + b) * kPG_N + threadIdx.x];
-  sums[(size_t)p * kPG_N + threadIdx.x] = s;
+  double *o = sums + (size_t)p * kPG_N + threadIdx.x;
+  *o = accumulate ? *o + s : s;
 }
 
 // dX/d(du, dv, dw) = R^T dX/dc; dX/d(a, b, c) = <dX/dR, R G> with G the
@@ -104,9 +109,8 @@ __global__ void kf_adjoint_finish(FitArgs args, const double *sums, double *grad
 }  // namespace kfd
 
 extern "C" hipError_t kfd_launch_adjoint(const kfd::FitArgs *args, const kfd::AdjArgs *adj,
-                                         double *sums, double *grad_planes, double *grad_scale,
-                                         double *chi2, hipStream_t stream) {
-  const int64_t blocks = (args->n + KF_BLOCK - 1) / KF_BLOCK;
+                                         double *sums, int accumulate, hipStream_t stream) {
+  const int64_t blocks = (args->count + KF_BLOCK - 1) / KF_BLOCK;
   if (blocks > 0)
     hipLaunchKernelGGL(kfd::kf_adjoint_kernel, dim3((unsigned)blocks), dim3(KF_BLOCK), 0, stream,
                        *args, *adj);
@@ -114,9 +118,14 @@ extern "C" hipError_t kfd_launch_adjoint(const kfd::FitArgs *args, const kfd::Ad
   hipLaunchKernelGGL(kfd::kf_adjoint_reduce, dim3((unsigned)(args->n_planes + 2), kfd::kRedSlices),
                      dim3(kfd::kRedThreads), 0, stream, *args, *adj, partial);
   hipLaunchKernelGGL(kfd::kf_adjoint_reduce2, dim3((unsigned)(args->n_planes + 2)),
-                     dim3(kfd::kPG_N), 0, stream, (const double *)partial, sums);
+                     dim3(kfd::kPG_N), 0, stream, (const double *)partial, sums, accumulate);
+  return hipGetLastError();
+}
+
+extern "C" hipError_t kfd_launch_adjoint_finish(const kfd::FitArgs *args, const double *sums,
+                                                double *grad_planes, double *grad_scale,
+                                                double *chi2, hipStream_t stream) {
   hipLaunchKernelGGL(kfd::kf_adjoint_finish, dim3((unsigned)((args->n_planes + 63) / 64)),
-                     dim3(64), 0, stream, *args, (const double *)sums, grad_planes, grad_scale,
-                     chi2);
+                     dim3(64), 0, stream, *args, sums, grad_planes, grad_scale, chi2);
   return hipGetLastError();
 }

@@ -52,8 +52,10 @@ struct AdjArgs {
   double *stack;          // [chunk][max_seg_steps][kSR_N][64]
   int32_t max_seg_steps;  // stack depth per lane
   int32_t max_seq;        // plane-adjoint slots per track (longest sequence)
-  double *plane_adj;      // [max_seq][kPG_N][n]
-  double *scale_adj;      // [n]
+  // per-track adjoints, indexed by the position within the launch
+  // (FitArgs::first + local = track)
+  double *plane_adj;      // [max_seq][kPG_N][count]
+  double *scale_adj;      // [count]
   uint32_t *status;       // [n]: 0 ok, else KF_ADJ_* bits
 };
 
@@ -941,21 +943,22 @@ struct MaterialFwd {
 struct TrackAdjoint {
   const FitArgs &a;
   const AdjArgs &g;
-  const int64_t i;
+  const int64_t i;    // track
+  const int64_t loc;  // position within the launch
   double *stk;  // this wave's step stack
   uint32_t lane;
   TrackFit<double, true> tf;  // recomputes segments from the checkpoints
 
-  KFD_INLINE TrackAdjoint(const FitArgs &args, const AdjArgs &adj, int64_t idx, double *ckpt,
-                          double *stack, uint32_t ln)
-      : a(args), g(adj), i(idx), stk(stack), lane(ln),
+  KFD_INLINE TrackAdjoint(const FitArgs &args, const AdjArgs &adj, int64_t idx, int64_t local,
+                          double *ckpt, double *stack, uint32_t ln)
+      : a(args), g(adj), i(idx), loc(local), stk(stack), lane(ln),
         tf(args, idx, nullptr, nullptr, nullptr, ckpt, ln) {}
 
   KFD_INLINE double *sr(int j, int f) const {
     return stk + (uint32_t)(((uint32_t)j * (uint32_t)kSR_N + (uint32_t)f) * 64u + lane);
   }
   KFD_INLINE double *pg(int q, int f) const {
-    return g.plane_adj + ((size_t)q * kPG_N + f) * (size_t)a.n + i;
+    return g.plane_adj + ((size_t)q * kPG_N + f) * (size_t)a.count + loc;
   }
   KFD_INLINE void addPlane(int plane, const double cb[3], const double Rb[3][3]) const {
     const int q = plane - tf.seqFirst;
@@ -983,7 +986,7 @@ struct TrackAdjoint {
     uint32_t st = 0;
     double sb = 0.0;
     if (nmeas <= 0) {
-      g.scale_adj[i] = 0.0;
+      g.scale_adj[loc] = 0.0;
       if (g.status) g.status[i] = KF_ADJ_SKIPPED;
       return;
     }
@@ -1194,7 +1197,7 @@ struct TrackAdjoint {
         for (int f = 0; f < kPG_N; ++f) *pg(q, f) = 0.0;
       sb = 0.0;
     }
-    g.scale_adj[i] = sb;
+    g.scale_adj[loc] = sb;
     if (g.status) g.status[i] = st;
   }
 };

@@ -34,6 +34,7 @@ struct kf_ctx {
   void *adj = nullptr;      // chi2-adjoint buffers (step stack, plane adjoints, ...)
   size_t adj_bytes = 0;
   bool has_material = false;  // a plane of the table carries material
+  uint64_t budget = KF_DEFAULT_MEMORY_BUDGET;  // per-batch working buffers
   hipEvent_t ev0 = nullptr, evm = nullptr, ev1 = nullptr;
   bool timed = false;
   bool phase_timed = false;  // evm was recorded (not under graph replay)
@@ -293,11 +294,32 @@ int setupArgs(kf_ctx *ctx, const kf_tracks *tr, const kf_options *opt, const cha
   return KF_OK;
 }
 
+constexpr size_t kCarryBytes = 6 * sizeof(int32_t) + 30 * sizeof(double);
+
+// Working-buffer bytes per track of the fit: per-state stores and hand-over.
+size_t storeBytesPerTrack(const kfd::FitArgs &a, int precision, bool track_states,
+                          bool checkpoints) {
+  const size_t s_size = (precision == KF_PRECISION_FP64) ? sizeof(double) : sizeof(float);
+  size_t per_state = KF_SD_N * sizeof(double) + KF_SS_N * s_size;
+  if (track_states) per_state += KF_TS_STRIDE * sizeof(double);
+  if (checkpoints) per_state += KF_CK_N * sizeof(double);
+  return (size_t)a.max_states * per_state + kCarryBytes;
+}
+
+// Positions per slice: the largest multiple of 64 whose working set fits the
+// budget (at least 64), or the whole batch.
+int64_t sliceTracks(const kf_ctx *ctx, int64_t n, size_t bytes_per_track) {
+  int64_t m = (int64_t)(ctx->budget / (bytes_per_track ? bytes_per_track : 1)) / 64 * 64;
+  if (m < 64) m = 64;
+  return m >= n ? n : m;
+}
+
 // Per-state stores (layout in kf_kernels.hpp) and the forward -> backward
-// hand-over, carved from the context's buffers.
-int setupStores(kf_ctx *ctx, int precision, bool track_states, bool checkpoints, hipStream_t s,
-                kfd::FitArgs &a) {
-  const size_t slots = ((size_t)a.n + 63) / 64 * 64 * (size_t)a.max_states;
+// hand-over for slices of up to `slice` positions, carved from the
+// context's buffers.
+int setupStores(kf_ctx *ctx, int precision, bool track_states, bool checkpoints, int64_t slice,
+                hipStream_t s, kfd::FitArgs &a) {
+  const size_t slots = ((size_t)slice + 63) / 64 * 64 * (size_t)a.max_states;
   const size_t s_size = (precision == KF_PRECISION_FP64) ? sizeof(double) : sizeof(float);
   const size_t d_bytes = slots * KF_SD_N * sizeof(double);
   const size_t s_bytes = (slots * KF_SS_N * s_size + 255) / 256 * 256;
@@ -311,12 +333,36 @@ int setupStores(kf_ctx *ctx, int precision, bool track_states, bool checkpoints,
   a.tsd = (double *)(base + s_bytes);
   a.tsf = track_states ? (double *)(base + s_bytes + d_bytes) : nullptr;
   a.tsc = checkpoints ? (double *)(base + s_bytes + d_bytes + f_bytes) : nullptr;
-  rc = grow(ctx, &ctx->carry, &ctx->carry_bytes,
-            (size_t)a.n * (6 * sizeof(int32_t) + 30 * sizeof(double)), s, "hand-over buffer");
+  rc = grow(ctx, &ctx->carry, &ctx->carry_bytes, (size_t)slice * kCarryBytes, s,
+            "hand-over buffer");
   if (rc != KF_OK) return rc;
+  // [field][count] of each slice (sliceCarry)
   a.carry_d = (double *)ctx->carry;
-  a.carry_i = (int32_t *)((char *)ctx->carry + (size_t)a.n * 30 * sizeof(double));
+  a.carry_i = nullptr;
   return KF_OK;
+}
+
+// Sets positions first .. first + count - 1 as the launch's range; the
+// hand-over record is [30 doubles][count] followed by [6 ints][count].
+void sliceRange(kfd::FitArgs &a, int64_t first, int64_t count) {
+  a.first = first;
+  a.count = count;
+  a.carry_i = (int32_t *)(a.carry_d + 30 * (size_t)count);
+}
+
+// The whole fit of a batch: the length sort over all positions, then the
+// positions slice by slice (each slice reuses the stores).  `between` is
+// recorded between the two kernels when there is one slice.
+hipError_t launchSlices(kfd::FitArgs a, const KfdLaunch &L, int precision, double *ts_out,
+                        int64_t slice, hipEvent_t between, hipStream_t s) {
+  hipError_t e = hipSuccess;
+  if (L.sort_by_length) e = kfd_launch_sort(&a, &L, s);
+  const bool one = slice >= a.n;
+  for (int64_t first = 0; e == hipSuccess && first < a.n; first += slice) {
+    sliceRange(a, first, (a.n - first < slice) ? a.n - first : slice);
+    e = kfd_launch_fit(&a, precision, s, ts_out, one ? between : nullptr, &L);
+  }
+  return e;
 }
 
 }  // namespace
@@ -343,7 +389,9 @@ int kf_fit(kf_ctx *ctx, const kf_tracks *tr, const kf_options *opt, const kf_res
   a.out_status = res->status;
   KF_HIP(ctx, hipSetDevice(ctx->device));
   hipStream_t s = (hipStream_t)stream;
-  rc = setupStores(ctx, opt->precision, res->track_states != nullptr, false, s, a);
+  const bool ts = res->track_states != nullptr;
+  const int64_t slice = sliceTracks(ctx, a.n, storeBytesPerTrack(a, opt->precision, ts, false));
+  rc = setupStores(ctx, opt->precision, ts, false, slice, s, a);
   if (rc != KF_OK) return rc;
   const kf_schedule &sc = ctx->sched;
   KfdLaunch L{};
@@ -367,12 +415,17 @@ int kf_fit(kf_ctx *ctx, const kf_tracks *tr, const kf_options *opt, const kf_res
   hipError_t e = hipSuccess;
   if (sc.graph) {
     // key: every argument the captured launches hold
-    std::vector<char> key(sizeof a + sizeof L + sizeof(int) + sizeof(void *));
+    std::vector<char> key(sizeof a + sizeof L + sizeof(int) + sizeof(void *) + sizeof slice);
     char *k = key.data();
     std::memcpy(k, &a, sizeof a);
-    std::memcpy(k + sizeof a, &L, sizeof L);
-    std::memcpy(k + sizeof a + sizeof L, &opt->precision, sizeof(int));
-    std::memcpy(k + sizeof a + sizeof L + sizeof(int), &res->track_states, sizeof(void *));
+    k += sizeof a;
+    std::memcpy(k, &L, sizeof L);
+    k += sizeof L;
+    std::memcpy(k, &opt->precision, sizeof(int));
+    k += sizeof(int);
+    std::memcpy(k, &res->track_states, sizeof(void *));
+    k += sizeof(void *);
+    std::memcpy(k, &slice, sizeof slice);
     if (!ctx->gexec || key != ctx->gkey) {
       if (ctx->gexec) (void)hipGraphExecDestroy(ctx->gexec);
       if (ctx->graph) (void)hipGraphDestroy(ctx->graph);
@@ -381,7 +434,7 @@ int kf_fit(kf_ctx *ctx, const kf_tracks *tr, const kf_options *opt, const kf_res
       ctx->gkey.clear();
       KF_HIP(ctx, hipStreamSynchronize(s));
       KF_HIP(ctx, hipStreamBeginCapture(ctx->cap_stream, hipStreamCaptureModeThreadLocal));
-      e = kfd_launch_fit(&a, opt->precision, ctx->cap_stream, res->track_states, nullptr, &L);
+      e = launchSlices(a, L, opt->precision, res->track_states, slice, nullptr, ctx->cap_stream);
       hipGraph_t g = nullptr;
       const hipError_t e2 = hipStreamEndCapture(ctx->cap_stream, &g);
       if (e == hipSuccess) e = e2;
@@ -398,8 +451,8 @@ int kf_fit(kf_ctx *ctx, const kf_tracks *tr, const kf_options *opt, const kf_res
     ctx->phase_timed = false;
   } else {
     KF_HIP(ctx, hipEventRecord(ctx->ev0, s));
-    e = kfd_launch_fit(&a, opt->precision, s, res->track_states, ctx->evm, &L);
-    ctx->phase_timed = true;
+    e = launchSlices(a, L, opt->precision, res->track_states, slice, ctx->evm, s);
+    ctx->phase_timed = slice >= a.n;
   }
   if (e != hipSuccess) return fail(ctx, KF_E_HIP, "kf_fit: launch failed: %s", hipGetErrorString(e));
   KF_HIP(ctx, hipEventRecord(ctx->ev1, s));
@@ -496,18 +549,25 @@ int kf_chi2_gradient(kf_ctx *ctx, const kf_tracks *tr, const kf_options *opt, do
   a.smoothing = 0;  // the objective is the forward filter's chi2
   KF_HIP(ctx, hipSetDevice(ctx->device));
   hipStream_t s = (hipStream_t)stream;
-  rc = setupStores(ctx, KF_PRECISION_FP64, false, true, s, a);
-  if (rc != KF_OK) return rc;
   const int32_t max_seg = KF_ADJ_MAX_SEGMENT_STEPS;
   const size_t n = (size_t)tr->n_tracks;
-  const size_t chunks = (n + 63) / 64;
   const size_t nplanes = (size_t)ctx->n_planes;
-  // forward outputs (params, cov, chi2, ndf, n_measured, status), step
-  // stack, per-track plane adjoints, field-scale adjoints, flags, sums
+  // per slice position: stores, step stack, plane adjoints, scale adjoint
+  const size_t adj_per_track = (size_t)max_seg * kfd::kSR_N * sizeof(double) +
+                               nplanes * kfd::kPG_N * sizeof(double) + sizeof(double);
+  const int64_t slice = sliceTracks(
+      ctx, a.n, storeBytesPerTrack(a, KF_PRECISION_FP64, false, true) + adj_per_track);
+  rc = setupStores(ctx, KF_PRECISION_FP64, false, true, slice, s, a);
+  if (rc != KF_OK) return rc;
+  const size_t sl = (size_t)(slice > 0 ? slice : 1);
+  const size_t chunks = (sl + 63) / 64;
+  // forward outputs of the batch (params, cov, chi2, ndf, n_measured,
+  // status) and flags; per slice: step stack, plane adjoints, field-scale
+  // adjoints; the sums
   const size_t b_out = n * (6 + 21 + 1) * sizeof(double) + n * 3 * sizeof(int32_t);
   const size_t b_stack = chunks * 64 * (size_t)max_seg * kfd::kSR_N * sizeof(double);
-  const size_t b_pg = nplanes * kfd::kPG_N * n * sizeof(double);
-  const size_t b_scale = n * sizeof(double);
+  const size_t b_pg = nplanes * kfd::kPG_N * sl * sizeof(double);
+  const size_t b_scale = sl * sizeof(double);
   const size_t b_flags = n * sizeof(uint32_t);
   // sums [n_planes + 2][12] followed by the reduction's partials
   const size_t b_sums = (nplanes + 2) * kfd::kPG_N * (1 + KF_ADJ_RED_SLICES) * sizeof(double);
@@ -537,12 +597,23 @@ int kf_chi2_gradient(kf_ctx *ctx, const kf_tracks *tr, const kf_options *opt, do
   p += al(b_flags);
   double *sums = (double *)p;
   KF_HIP(ctx, hipEventRecord(ctx->ev0, s));
-  hipError_t e =
-      (n > 0) ? kfd_launch_fit(&a, KF_PRECISION_FP64, s, nullptr, ctx->evm, nullptr) : hipSuccess;
-  ctx->phase_timed = true;
-  if (e == hipSuccess && n == 0) e = hipEventRecord(ctx->evm, s);
+  // per slice: forward filter with checkpoints, then the reverse sweep and
+  // its reduction onto the sums of the earlier slices
+  hipError_t e = hipSuccess;
+  if (n == 0) {
+    sliceRange(a, 0, 0);
+    e = hipEventRecord(ctx->evm, s);
+    if (e == hipSuccess) e = kfd_launch_adjoint(&a, &g, sums, 0, s);
+  }
+  for (int64_t first = 0; e == hipSuccess && first < a.n; first += slice) {
+    sliceRange(a, first, (a.n - first < slice) ? a.n - first : slice);
+    e = kfd_launch_fit(&a, KF_PRECISION_FP64, s, nullptr,
+                       slice >= a.n ? ctx->evm : nullptr, nullptr);
+    if (e == hipSuccess) e = kfd_launch_adjoint(&a, &g, sums, first > 0, s);
+  }
+  ctx->phase_timed = slice >= a.n;
   if (e == hipSuccess)
-    e = kfd_launch_adjoint(&a, &g, sums, grad_planes, grad_field_scale, chi2_sum, s);
+    e = kfd_launch_adjoint_finish(&a, sums, grad_planes, grad_field_scale, chi2_sum, s);
   if (e == hipSuccess && grad_raw)
     e = hipMemcpyAsync(grad_raw, sums, nplanes * kfd::kPG_N * sizeof(double),
                        hipMemcpyDeviceToDevice, s);
@@ -550,6 +621,12 @@ int kf_chi2_gradient(kf_ctx *ctx, const kf_tracks *tr, const kf_options *opt, do
     return fail(ctx, KF_E_HIP, "kf_chi2_gradient: launch failed: %s", hipGetErrorString(e));
   KF_HIP(ctx, hipEventRecord(ctx->ev1, s));
   ctx->timed = true;
+  return KF_OK;
+}
+
+int kf_set_memory_budget(kf_ctx *ctx, uint64_t bytes) {
+  if (!ctx) return fail(nullptr, KF_E_INVALID, "kf_set_memory_budget: ctx is NULL");
+  ctx->budget = bytes ? bytes : KF_DEFAULT_MEMORY_BUDGET;
   return KF_OK;
 }
 

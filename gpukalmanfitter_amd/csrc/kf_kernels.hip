@@ -27,11 +27,12 @@ struct ChunkBases {
 
 // Fits the track at position pos (pass FWD of the two kernels).
 template <typename S, bool FWD>
-KFD_INLINE void fitPosition(const FitArgs &a, int64_t pos, const DevPlane *planes) {
-  const ChunkBases<S> cb(a, pos);
-  if (pos >= a.n) return;
+KFD_INLINE void fitPosition(const FitArgs &a, int64_t local, const DevPlane *planes) {
+  const ChunkBases<S> cb(a, local);
+  if (local >= a.count) return;
+  const int64_t pos = a.first + local;
   const int64_t i = a.order ? (int64_t)a.order[pos] : pos;
-  TrackFit<S, FWD> tf(a, i, cb.d, cb.s, cb.f, cb.c, (uint32_t)(pos & 63));
+  TrackFit<S, FWD> tf(a, i, cb.d, cb.s, cb.f, cb.c, (uint32_t)(local & 63), local);
   tf.pls = planes;
   if constexpr (FWD)
     tf.runForward();
@@ -58,7 +59,7 @@ __global__ void __launch_bounds__(KF_BLOCK) kf_fit_kernel(FitArgs args) {
     planes = lds_planes;
   }
   const uint32_t lane = threadIdx.x & 63u;
-  const uint32_t nchunks = (uint32_t)((args.n + 63) >> 6);
+  const uint32_t nchunks = (uint32_t)((args.count + 63) >> 6);
   const uint32_t grab = (uint32_t)args.chunks_per_grab;
   uint32_t *q = args.queue + (FWD ? 0 : 1);
   // one-shot grid: this wave's chunk; persistent: chunks come from the queue
@@ -81,13 +82,14 @@ __global__ void __launch_bounds__(KF_BLOCK) kf_fit_kernel(FitArgs args) {
 // Copies the public track-state records into kf_results.track_states
 // ([state][KF_TS_STRIDE][n]).
 __global__ void __launch_bounds__(256) kf_export_states(FitArgs args, double *out) {
-  const int64_t pos = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (pos >= args.n) return;
+  const int64_t local = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (local >= args.count) return;
+  const int64_t pos = args.first + local;
   const int64_t i = args.order ? (int64_t)args.order[pos] : pos;
   const int nm = args.out_nmeas[i];
   const double *base =
-      args.tsf + (size_t)(pos >> 6) * (size_t)args.max_states * KF_TS_STRIDE * 64u;
-  const uint32_t lane = (uint32_t)(pos & 63);
+      args.tsf + (size_t)(local >> 6) * (size_t)args.max_states * KF_TS_STRIDE * 64u;
+  const uint32_t lane = (uint32_t)(local & 63);
   for (int k = 0; k < nm && k < args.max_states; ++k)
     for (int f = 0; f < KF_TS_STRIDE; ++f)
       out[((size_t)k * KF_TS_STRIDE + f) * (size_t)args.n + i] =
@@ -106,11 +108,15 @@ KFD_INLINE int lengthBin(const FitArgs &a, int64_t i) {
   return l < 0 ? 0 : (l >= kLenBins ? kLenBins - 1 : l);
 }
 
-// zeroes the histogram and bin offsets and the work-queue counters
-__global__ void __launch_bounds__(256) kf_sched_reset(uint32_t *scratch, uint32_t *queue) {
+// zeroes the histogram and bin offsets
+__global__ void __launch_bounds__(256) kf_sort_reset(uint32_t *scratch) {
   scratch[threadIdx.x] = 0u;
   scratch[kLenBins + threadIdx.x] = 0u;
-  if (queue && threadIdx.x < 2) queue[threadIdx.x] = 0u;
+}
+
+// zeroes the work-queue counters (forward, backward)
+__global__ void __launch_bounds__(64) kf_queue_reset(uint32_t *queue) {
+  if (threadIdx.x < 2) queue[threadIdx.x] = 0u;
 }
 
 __global__ void __launch_bounds__(256) kf_len_hist(FitArgs a, uint32_t *hist) {
@@ -157,22 +163,11 @@ __global__ void __launch_bounds__(256) kf_len_scatter(FitArgs a, uint32_t *off, 
 extern "C" hipError_t kfd_launch_fit(const kfd::FitArgs *args, int precision, hipStream_t stream,
                                      double *track_states_out, hipEvent_t between,
                                      const KfdLaunch *launch) {
-  if (args->n <= 0) return hipSuccess;
-  const bool sort = launch && launch->sort_by_length;
+  if (args->count <= 0) return hipSuccess;
   const bool pers = launch && launch->persistent_blocks > 0;
-  if (sort || pers) {
-    hipLaunchKernelGGL(kfd::kf_sched_reset, dim3(1), dim3(256), 0, stream, launch->sort_scratch,
-                       args->queue);
-  }
-  if (sort) {
-    const unsigned sb = (unsigned)((args->n + 255) / 256);
-    uint32_t *hist = launch->sort_scratch, *off = launch->sort_scratch + kfd::kLenBins;
-    hipLaunchKernelGGL(kfd::kf_len_hist, dim3(sb), dim3(256), 0, stream, *args, hist);
-    hipLaunchKernelGGL(kfd::kf_len_scan, dim3(1), dim3(256), 0, stream, hist, off);
-    hipLaunchKernelGGL(kfd::kf_len_scatter, dim3(sb), dim3(256), 0, stream, *args, off,
-                       launch->order);
-  }
-  const int64_t blocks = pers ? launch->persistent_blocks : (args->n + KF_BLOCK - 1) / KF_BLOCK;
+  if (pers) hipLaunchKernelGGL(kfd::kf_queue_reset, dim3(1), dim3(64), 0, stream, args->queue);
+  const int64_t blocks =
+      pers ? launch->persistent_blocks : (args->count + KF_BLOCK - 1) / KF_BLOCK;
   const dim3 grid((unsigned)blocks), block(KF_BLOCK);
   const bool lds = args->n_planes <= KF_LDS_PLANES;
   const bool fp64 = precision == KF_PRECISION_FP64;
@@ -197,9 +192,22 @@ extern "C" hipError_t kfd_launch_fit(const kfd::FitArgs *args, int precision, hi
     KFD_LAUNCH(float, false);
 #undef KFD_LAUNCH
   if (track_states_out) {
-    const int64_t eb = (args->n + 255) / 256;
+    const int64_t eb = (args->count + 255) / 256;
     hipLaunchKernelGGL(kfd::kf_export_states, dim3((unsigned)eb), dim3(256), 0, stream, *args,
                        track_states_out);
   }
+  return hipGetLastError();
+}
+
+extern "C" hipError_t kfd_launch_sort(const kfd::FitArgs *args, const KfdLaunch *launch,
+                                      hipStream_t stream) {
+  if (args->n <= 0) return hipSuccess;
+  const unsigned sb = (unsigned)((args->n + 255) / 256);
+  uint32_t *hist = launch->sort_scratch, *off = launch->sort_scratch + kfd::kLenBins;
+  hipLaunchKernelGGL(kfd::kf_sort_reset, dim3(1), dim3(256), 0, stream, launch->sort_scratch);
+  hipLaunchKernelGGL(kfd::kf_len_hist, dim3(sb), dim3(256), 0, stream, *args, hist);
+  hipLaunchKernelGGL(kfd::kf_len_scan, dim3(1), dim3(256), 0, stream, hist, off);
+  hipLaunchKernelGGL(kfd::kf_len_scatter, dim3(sb), dim3(256), 0, stream, *args, off,
+                     launch->order);
   return hipGetLastError();
 }

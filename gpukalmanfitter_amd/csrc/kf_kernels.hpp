@@ -38,7 +38,10 @@ namespace kfd {
 
 // Kernel argument block (passed by value).  All pointers are device memory.
 struct FitArgs {
-  int64_t n;
+  int64_t n;  // tracks in the batch (the stride of the [component][n] arrays)
+  // this launch covers track positions first .. first + count - 1; the
+  // per-state stores are indexed by the position within the launch
+  int64_t first, count;
   int64_t n_hits;
   const double *start_params;
   const double *start_cov;
@@ -110,11 +113,16 @@ struct FitArgs {
 namespace kfd {
 struct AdjArgs;
 }
-// Launches the reverse sweep, the reduction and the conversion of the chi2
-// adjoint (kf_adjoint.hip); the forward filter with checkpoints must have run.
+// Launches the reverse sweep of the chi2 adjoint over the positions
+// args->first .. first + count - 1 and its reduction into sums (added onto
+// the previous launches' sums when accumulate != 0); the forward filter with
+// checkpoints must have run over the same positions.
 extern "C" hipError_t kfd_launch_adjoint(const kfd::FitArgs *args, const kfd::AdjArgs *adj,
-                                         double *sums, double *grad_planes, double *grad_scale,
-                                         double *chi2, hipStream_t stream);
+                                         double *sums, int accumulate, hipStream_t stream);
+// Converts the summed raw adjoints to per-plane alignment gradients.
+extern "C" hipError_t kfd_launch_adjoint_finish(const kfd::FitArgs *args, const double *sums,
+                                                double *grad_planes, double *grad_scale,
+                                                double *chi2, hipStream_t stream);
 
 // Launch shape of kfd_launch_fit.
 struct KfdLaunch {
@@ -124,11 +132,17 @@ struct KfdLaunch {
   int32_t *order;          // [n] positions -> tracks (when sorting)
 };
 
-// Launches the fit: [length sort] -> forward kernel -> record `between` (if
-// not NULL) -> smoother/back-propagation kernel -> track-state export (if
+// Launches the fit of the positions args->first .. first + count - 1:
+// [work-queue reset] -> forward kernel -> record `between` (if not NULL) ->
+// smoother/back-propagation kernel -> track-state export (if
 // track_states_out is not NULL).  `launch` may be NULL (one thread per
-// track, identity order).  With sorting or a persistent grid, args->order,
-// args->queue and args->chunks_per_grab must be set.
+// track, identity order).  With a persistent grid, args->queue and
+// args->chunks_per_grab must be set; with sorting, args->order.
 extern "C" hipError_t kfd_launch_fit(const kfd::FitArgs *args, int precision, hipStream_t stream,
                                      double *track_states_out, hipEvent_t between,
                                      const KfdLaunch *launch);
+
+// The length sort of the whole batch (positions -> tracks into launch->order),
+// before the per-slice kfd_launch_fit calls.
+extern "C" hipError_t kfd_launch_sort(const kfd::FitArgs *args, const KfdLaunch *launch,
+                                      hipStream_t stream);

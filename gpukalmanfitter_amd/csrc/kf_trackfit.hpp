@@ -74,11 +74,14 @@ struct TrackFit {
   // the plane table: a copy in LDS (set by the kernel) or a.planes
   const DevPlane *pls;
   uint32_t lane;
+  // slot of the forward -> backward hand-over record: the position within
+  // the launch, so consecutive lanes use consecutive slots
+  int64_t cpos;
 
   KFD_INLINE TrackFit(const FitArgs &args, int64_t idx, double *d_base, S *s_base, double *f_base,
-                      double *c_base, uint32_t ln)
+                      double *c_base, uint32_t ln, int64_t carry_slot = 0)
       : a(args), i(idx), tsd(d_base), tss(s_base), tsf(f_base), tsc(c_base), pls(args.planes),
-        lane(ln) {}
+        lane(ln), cpos(carry_slot) {}
 
   // ------------------------------------------------------------ storage ----
   // Element f of state k: base[(k * F + f) * 64 + lane] -- a 32-bit lane
@@ -1191,8 +1194,8 @@ struct TrackFit {
   // Per-track hand-over record from the forward to the backward kernel.
   enum { kCI_resume = 0, kCI_steps, kCI_measured, kCI_first, kCI_next, kCI_flags, kCI_N };
   enum { kCD_q = 0, kCD_J = 1, kCD_k4 = 26, kCD_dtds = 29, kCD_N = 30 };
-  KFD_INLINE int32_t &ci(int f) const { return a.carry_i[(size_t)f * a.n + i]; }
-  KFD_INLINE double &cd(int f) const { return a.carry_d[(size_t)f * a.n + i]; }
+  KFD_INLINE int32_t &ci(int f) const { return a.carry_i[(size_t)f * a.count + cpos]; }
+  KFD_INLINE double &cd(int f) const { return a.carry_d[(size_t)f * a.count + cpos]; }
 
   KFD_INLINE void saveCarry(int resume, int steps, bool terminated) const {
     ci(kCI_resume) = resume;

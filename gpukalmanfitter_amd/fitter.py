@@ -67,6 +67,8 @@ def _lib():
                                     C.POINTER(abi.kf_results), P, C.c_int32,
                                     C.POINTER(abi.kf_schedule), C.POINTER(C.c_double)]
         lib.kf_autotune.restype = C.c_int
+        lib.kf_set_memory_budget.argtypes = [P, C.c_uint64]
+        lib.kf_set_memory_budget.restype = C.c_int
         lib._kf_typed = True
     return lib
 
@@ -301,6 +303,11 @@ class KalmanFitter:
         for k, v in fields.items():
             setattr(s, k, int(v))
         self._check(self._lib.kf_set_schedule(self._ctx, C.byref(s)))
+
+    def set_memory_budget(self, nbytes):
+        """Device-memory budget of the per-batch working buffers (0: the
+        library default, 16 GiB); larger batches run in slices."""
+        self._check(self._lib.kf_set_memory_budget(self._ctx, int(nbytes)))
 
     def schedule(self):
         s = abi.kf_schedule()

@@ -241,6 +241,18 @@ int kf_autotune(kf_ctx *ctx, const kf_tracks *tracks, const kf_options *opt,
                 const kf_results *res, void *stream, int32_t repeats, kf_schedule *best,
                 double *best_ms);
 
+/* Device-memory budget of ctx's per-batch working buffers (per-state stores,
+ * kernel hand-over, adjoint stacks and per-track adjoints); 0 restores the
+ * default, KF_DEFAULT_MEMORY_BUDGET.  A batch whose working set exceeds the
+ * budget is fitted in consecutive slices of positions (multiples of 64
+ * tracks) that reuse the same buffers; results do not depend on the slicing.
+ * Buffers only grow: lowering the budget does not release memory (a new
+ * context does).  The reference divides a batch into nStreams equal parts
+ * for copy/compute overlap (Run/GPU/KalmanFitterGPUTest.cu:212-219); here the
+ * parts are sized by memory. */
+#define KF_DEFAULT_MEMORY_BUDGET (16ull << 30)
+int kf_set_memory_budget(kf_ctx *ctx, uint64_t bytes);
+
 /* Duration of the last fit kernel on ctx in milliseconds (HIP events recorded
  * on the launch stream); < 0 if unknown. */
 double kf_last_kernel_ms(const kf_ctx *ctx);

@@ -42,6 +42,8 @@ struct Host {
     for (int32_t p = 0; p < np; ++p) planes.push_back(devPlane(pl[p]));
     const int64_t n = tr->n_tracks;
     a.n = n;
+    a.first = 0;
+    a.count = n;
     a.n_hits = tr->n_hits;
     a.start_params = tr->start_params;
     a.start_cov = tr->start_cov;
@@ -108,7 +110,7 @@ struct Host {
       const size_t ch = (size_t)(i >> 6);
       TrackFit<double, true> tf(a, i, a.tsd + ch * per * KF_SD_N,
                                 (double *)a.tss + ch * per * KF_SS_N, nullptr,
-                                a.tsc + ch * per * KF_CK_N, (uint32_t)(i & 63));
+                                a.tsc + ch * per * KF_CK_N, (uint32_t)(i & 63), i);
       tf.runForward();
     }
   }
@@ -148,7 +150,7 @@ int kfh_chi2_gradient(const kf_plane *planes, int32_t n_planes, const kfh_field 
   g.status = flags;
   for (int64_t i = 0; i < n; ++i) {
     const size_t ch = (size_t)(i >> 6);
-    TrackAdjoint ta(h.a, g, i, h.a.tsc + ch * h.per * KF_CK_N,
+    TrackAdjoint ta(h.a, g, i, i, h.a.tsc + ch * h.per * KF_CK_N,
                     stack.data() + ch * (size_t)cap * kSR_N * 64, (uint32_t)(i & 63));
     ta.run();
   }

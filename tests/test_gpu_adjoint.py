@@ -195,6 +195,29 @@ def test_adjoint_behind_torch_autograd(torch_cuda):
     assert abs(float(scale.grad) - fds) <= TOL * abs(fds)
 
 
+def test_adjoint_memory_slices(torch_cuda):
+    """A memory budget of about 1,350 tracks splits the forward filter and
+    the reverse sweep into 15 slices whose sums add in slice order: the same
+    gradient as one slice up to summation order, the same objective and
+    flags."""
+    from gpukalmanfitter_amd.fitter import DeviceBatch, KalmanFitter, KalmanFitterOptions
+    b = G.generate(5, n_tracks=20000)
+    kf = KalmanFitter(b.planes, b.field)
+    dev = DeviceBatch(b)
+    opts = KalmanFitterOptions(**configs.WORKLOAD_FIT_OPTIONS)
+    grad, gscale, chi2, flags, raw = kf.chi2_gradient(dev, opts)
+    kf.set_memory_budget(64 << 20)
+    grad2, gscale2, chi22, flags2, raw2 = kf.chi2_gradient(dev, opts)
+    kf.close()
+    assert np.array_equal(flags, flags2)
+    assert abs(chi22 - chi2) <= 1e-12 * abs(chi2)
+    assert abs(gscale2 - gscale) <= 1e-10 * abs(gscale)
+    for j in range(6):
+        scale = np.abs(grad[:, j]).max()
+        assert np.abs(grad2[:, j] - grad[:, j]).max() <= 1e-10 * scale, j
+    assert np.abs(raw2 - raw).max() <= 1e-10 * np.abs(raw).max()
+
+
 def test_adjoint_material_config5(torch_cuda):
     """Config 5 sample (300 um silicon on every plane, multiple scattering and
     energy loss on, grid field): the material update is differentiated too."""

@@ -343,6 +343,31 @@ def test_schedules_give_identical_results(torch_cuda, sched):
     check_parity(ref, o)
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("sched", [dict(), SCHEDULES[2]])
+def test_memory_slices_give_identical_results(torch_cuda, precision, sched):
+    """A memory budget below the batch's working set splits the fit into
+    slices of positions (here 7-8 of at most 832 tracks, and the 64-track
+    minimum); every track's result is bitwise that of the one-slice fit."""
+    from gpukalmanfitter_amd.fitter import DeviceBatch, KalmanFitter, KalmanFitterOptions
+    b = G.generate(3, n_tracks=5000)
+    kf = KalmanFitter(b.planes, b.field)
+    kf.set_schedule(**sched)
+    dev = DeviceBatch(b, track_states=True)
+    opts = KalmanFitterOptions(precision=precision, **W)
+    kf.fit_batch(dev, opts)
+    ref = dev.results_numpy()
+    for budget in (32 << 20, 1):
+        kf.set_memory_budget(budget)
+        for _ in range(2):  # graph: capture, then a replay
+            dev.clear_results()
+            kf.fit_batch(dev, opts)
+            got = dev.results_numpy()
+            for k in ("params", "cov", "chi2", "status", "n_measured", "track_states"):
+                assert np.array_equal(got[k], ref[k]), (budget, k)
+    kf.close()
+
+
 def test_autotuner_on_ragged_batch(torch_cuda):
     """The autotuner times its candidate schedules on the batch and keeps the
     fastest; the kept schedule reproduces the default results."""
