@@ -46,7 +46,10 @@ KFD_INLINE void fitPosition(const FitArgs &a, int64_t local, const DevPlane *pla
 //
 // LDS = true: the block first copies the plane table (at most KF_LDS_PLANES
 // planes) to LDS and reads planes there.
-template <typename S, bool FWD, bool LDS>
+// PERS = false: one thread per position; true: the persistent work queue
+// (a separate instantiation, so the one-shot kernel does not carry the
+// queue loop's registers).
+template <typename S, bool FWD, bool LDS, bool PERS>
 __global__ void __launch_bounds__(KF_BLOCK) kf_fit_kernel(FitArgs args) {
   const DevPlane *planes = args.planes;
   if constexpr (LDS) {
@@ -59,23 +62,20 @@ __global__ void __launch_bounds__(KF_BLOCK) kf_fit_kernel(FitArgs args) {
     planes = lds_planes;
   }
   const uint32_t lane = threadIdx.x & 63u;
-  const uint32_t nchunks = (uint32_t)((args.count + 63) >> 6);
-  const uint32_t grab = (uint32_t)args.chunks_per_grab;
-  uint32_t *q = args.queue + (FWD ? 0 : 1);
-  // one-shot grid: this wave's chunk; persistent: chunks come from the queue
-  uint32_t c = blockIdx.x * (KF_BLOCK / 64) + (threadIdx.x >> 6);
-  uint32_t c1 = c + 1;
-  for (;;) {
-    if (args.persistent) {
+  if constexpr (!PERS) {
+    fitPosition<S, FWD>(args, (int64_t)blockIdx.x * KF_BLOCK + threadIdx.x, planes);
+  } else {
+    const uint32_t nchunks = (uint32_t)((args.count + 63) >> 6);
+    const uint32_t grab = (uint32_t)args.chunks_per_grab;
+    uint32_t *q = args.queue + (FWD ? 0 : 1);
+    for (;;) {
       uint32_t c0 = 0;
       if (lane == 0) c0 = atomicAdd(q, grab);
       c0 = (uint32_t)__shfl((int)c0, 0, 64);
       if (c0 >= nchunks) break;
-      c = c0;
-      c1 = (c0 + grab < nchunks) ? c0 + grab : nchunks;
+      const uint32_t c1 = (c0 + grab < nchunks) ? c0 + grab : nchunks;
+      for (uint32_t c = c0; c < c1; ++c) fitPosition<S, FWD>(args, (int64_t)c * 64 + lane, planes);
     }
-    for (; c < c1; ++c) fitPosition<S, FWD>(args, (int64_t)c * 64 + lane, planes);
-    if (!args.persistent) break;
   }
 }
 
@@ -171,12 +171,21 @@ extern "C" hipError_t kfd_launch_fit(const kfd::FitArgs *args, int precision, hi
   const dim3 grid((unsigned)blocks), block(KF_BLOCK);
   const bool lds = args->n_planes <= KF_LDS_PLANES;
   const bool fp64 = precision == KF_PRECISION_FP64;
-#define KFD_LAUNCH(S_, FWD_)                                                                  \
-  do {                                                                                        \
-    if (lds)                                                                                  \
-      hipLaunchKernelGGL((kfd::kf_fit_kernel<S_, FWD_, true>), grid, block, 0, stream, *args);  \
-    else                                                                                      \
-      hipLaunchKernelGGL((kfd::kf_fit_kernel<S_, FWD_, false>), grid, block, 0, stream, *args); \
+#define KFD_LAUNCH1(S_, FWD_, LDS_)                                                          \
+  do {                                                                                      \
+    if (pers)                                                                               \
+      hipLaunchKernelGGL((kfd::kf_fit_kernel<S_, FWD_, LDS_, true>), grid, block, 0, stream,  \
+                         *args);                                                            \
+    else                                                                                    \
+      hipLaunchKernelGGL((kfd::kf_fit_kernel<S_, FWD_, LDS_, false>), grid, block, 0, stream, \
+                         *args);                                                            \
+  } while (0)
+#define KFD_LAUNCH(S_, FWD_)          \
+  do {                                \
+    if (lds)                          \
+      KFD_LAUNCH1(S_, FWD_, true);    \
+    else                              \
+      KFD_LAUNCH1(S_, FWD_, false);   \
   } while (0)
   if (fp64)
     KFD_LAUNCH(double, true);
@@ -191,6 +200,7 @@ extern "C" hipError_t kfd_launch_fit(const kfd::FitArgs *args, int precision, hi
   else
     KFD_LAUNCH(float, false);
 #undef KFD_LAUNCH
+#undef KFD_LAUNCH1
   if (track_states_out) {
     const int64_t eb = (args->count + 255) / 256;
     hipLaunchKernelGGL(kfd::kf_export_states, dim3((unsigned)eb), dim3(256), 0, stream, *args,

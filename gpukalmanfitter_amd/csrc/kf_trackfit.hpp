@@ -36,7 +36,10 @@ struct TrackFit {
   int navDir;
   CStep ss;
   double pathAcc;
-  double cov[21];
+  // bound covariance (packed upper triangle).  S: in the mixed mode every
+  // consumer of it between surfaces (the next sandwich, the stores) takes it
+  // in float, so it is kept in float (half the registers)
+  S cov[21];
   // free-from-bound Jacobian J (8x6).  Non-constant entries only:
   // A = J[0:3, 0:5], b = J[3, 4], G = J[4:7, 2:5]; J[3,5] = J[7,4] = 1, the
   // rest is zero between surfaces.
@@ -64,6 +67,26 @@ struct TrackFit {
   double lc[3];  // perigee line point
   double fpar[6];
   double fcov[21];
+
+  // quantities of the momentum used by every step, recomputed when p changes:
+  // q/p, hypot(1, m/p) (dt/ds) and p hypot(1, m/p)
+  double cacheP = -1.0, cacheQ = 0.0, cacheQop = 0.0, cacheDtds = 0.0, cacheDen = 0.0;
+  KFD_INLINE void momentumCache() {
+    if (p != cacheP || q != cacheQ) {
+      cacheP = p;
+      cacheQ = q;
+      cacheQop = q / p;
+      cacheDtds = hypot(1.0, a.mass / p);
+      cacheDen = p * cacheDtds;
+    }
+  }
+  // the last plane intersection computed by surfaceStatus (path length along
+  // the direction, denominator n.d): navStatus and navTarget of one
+  // iteration usually ask for the same plane; valid until pos, dir or navDir
+  // change (moved())
+  int pathPlane = -1;
+  double pathLen = 0.0, pathDenom = 0.0;
+  KFD_INLINE void moved() { pathPlane = -1; }
 
   // per-state stores of this wave's 64-track chunk (wave-uniform bases) and
   // this lane's slot in them
@@ -134,6 +157,7 @@ struct TrackFit {
     for (int m = 0; m < 4; ++m) ss.v[m] = *recC(k, KF_CK_SS + m);
     ss.dir = 1;
     navDir = 1;
+    moved();
 #pragma unroll
     for (int m = 0; m < 21; ++m) cov[m] = *recC(k, KF_CK_COV + m);
 #pragma unroll
@@ -151,11 +175,12 @@ struct TrackFit {
     dtds = (S)0;
   }
   // public record: parameters + covariance at offset off
-  KFD_INLINE void storeBoundF(int k, int off, const double par[6], const double c[21]) const {
+  template <typename T>
+  KFD_INLINE void storeBoundF(int k, int off, const double par[6], const T c[21]) const {
 #pragma unroll
     for (int m = 0; m < 6; ++m) *recF(k, off + m) = par[m];
 #pragma unroll
-    for (int m = 0; m < 21; ++m) *recF(k, off + 6 + m) = c[m];
+    for (int m = 0; m < 21; ++m) *recF(k, off + 6 + m) = (double)c[m];
   }
   // packed index of Jacobian entry (r, c), r in {0, 1, 2, 3, 5}, c < 5
   KFD_INLINE static constexpr int jidx(int r, int c) { return ((r < 4) ? r : 4) * 5 + c; }
@@ -215,6 +240,7 @@ struct TrackFit {
     q = charge;
     t = pc[5];
     navDir = 1;
+    moved();
     ss.init(1.0 * fabs(a.max_step_size));
     pathAcc = 0.0;
     // line initJacobianToGlobal at (pos, dir) with the corrected parameters
@@ -271,7 +297,9 @@ struct TrackFit {
   KFD_INLINE bool step() {
     double B1[3], Bm[3], Bl[3];
     field(pos, B1);
-    const double qop = q / p;
+    moved();
+    momentumCache();
+    const double qop = cacheQop;
     double k1[3], k2[3], k3[3], k4[3];
     cross(dir, B1, k1);
 #pragma unroll
@@ -328,17 +356,21 @@ struct TrackFit {
     }
     const double h = ss.get();
     // propagationTime, EigenStepper.ipp:36-47
-    const double dt = hypot(1.0, a.mass / p);
+    const double dt = cacheDtds;
     t = t + h * dt;
     transport(B1, Bm, Bl, k1, k2, k3, h, qop);
+    // h^2/6 and h/6 as products with 1/6, one division for the
+    // normalisation (within an ulp of the quotients)
+    const double h6 = h * (1.0 / 6.0);
+    const double hh6 = h * h6;
 #pragma unroll
-    for (int k = 0; k < 3; ++k) pos[k] = pos[k] + h * dir[k] + (h * h / 6.0) * (k1[k] + k2[k] + k3[k]);
+    for (int k = 0; k < 3; ++k) pos[k] = pos[k] + h * dir[k] + hh6 * (k1[k] + k2[k] + k3[k]);
 #pragma unroll
     for (int k = 0; k < 3; ++k)
-      dir[k] = dir[k] + (h / 6.0) * (k1[k] + 2.0 * (k2[k] + k3[k]) + k4[k]);
-    const double dn = sqrt(dir[0] * dir[0] + dir[1] * dir[1] + dir[2] * dir[2]);
+      dir[k] = dir[k] + h6 * (k1[k] + 2.0 * (k2[k] + k3[k]) + k4[k]);
+    const double idn = 1.0 / sqrt(dir[0] * dir[0] + dir[1] * dir[1] + dir[2] * dir[2]);
 #pragma unroll
-    for (int k = 0; k < 3; ++k) dir[k] = dir[k] / dn;
+    for (int k = 0; k < 3; ++k) dir[k] = dir[k] * idn;
     derivSet = true;
     dtds = (S)dt;
 #pragma unroll
@@ -397,8 +429,8 @@ struct TrackFit {
     crossStep(dk2dT, half_h, Bm, qop, dk3dT);
     crossStep(dk3dT, h, Bl, qop, dk4dT);
 
-    const S h6 = h / (S)6.0;
-    const S hh6 = (h * h) / (S)6.0;
+    const S h6 = h * (S)(1.0 / 6.0);
+    const S hh6 = h * h6;
     S dFdT[3][3], dGdT[3][3], dFdL[3], dGdL[3];
 #pragma unroll
     for (int r = 0; r < 3; ++r) {
@@ -412,7 +444,7 @@ struct TrackFit {
       dGdL[r] = h6 * (dk1dL[r] + (S)2.0 * (dk2dL[r] + dk3dL[r]) + dk4dL[r]);
     }
     const double m = a.mass;
-    const S d37 = (S)(hd * m * m * q / (p * hypot(1.0, m / p)));
+    const S d37 = (S)(hd * m * m * q / cacheDen);
 
     // J <- D J on the non-constant entries
 #pragma unroll
@@ -480,7 +512,7 @@ struct TrackFit {
     }
     S cs[21];
 #pragma unroll
-    for (int m = 0; m < 21; ++m) cs[m] = (S)cov[m];
+    for (int m = 0; m < 21; ++m) cs[m] = cov[m];
     sandwich(jb, cs, cov);
   }
 
@@ -489,7 +521,8 @@ struct TrackFit {
   // skipped; row r of jb c is formed and consumed at once.  The smoother
   // calls this with the stored operands to recompute the predicted
   // covariance bit for bit.
-  KFD_INLINE static void sandwich(const S jb[6][6], const S c[21], double cn[21]) {
+  template <typename Out>
+  KFD_INLINE static void sandwich(const S jb[6][6], const S c[21], Out cn[21]) {
 #pragma unroll
     for (int r = 0; r < 6; ++r) {
       S T[6];
@@ -516,7 +549,7 @@ struct TrackFit {
           for (int m = 1; m < 5; ++m) s = s + T[m] * jb[cc][m];
           if (cc == 5) s = s + T[5];
         }
-        cn[sidx(r, cc)] = (double)s;
+        cn[sidx(r, cc)] = (Out)s;
       }
     }
   }
@@ -617,7 +650,7 @@ struct TrackFit {
   // EigenStepper::update from bound parameters on a plane
   // (boundParameters2freeParameters, coordinate_transformations.hpp:145-158,
   // then EigenStepper.ipp:554-564)
-  KFD_INLINE void updateFromBound(const DevPlane &pl, const double par[6], const double c[21]) {
+  KFD_INLINE void updateFromBound(const DevPlane &pl, const double par[6]) {
 #pragma unroll
     for (int k = 0; k < 3; ++k) pos[k] = (pl.r[3 * k] * par[0] + pl.r[3 * k + 1] * par[1]) + pl.c[k];
     const double cT = cos(par[3]), sT = sin(par[3]);
@@ -626,10 +659,9 @@ struct TrackFit {
     dir[0] = d0 / dn;
     dir[1] = d1 / dn;
     dir[2] = d2 / dn;
+    moved();
     p = fabs(1.0 / par[4]);
     t = par[5];
-#pragma unroll
-    for (int k = 0; k < 21; ++k) cov[k] = c[k];
   }
 
   // ----------------------------------------------------------- navigation ----
@@ -637,14 +669,24 @@ struct TrackFit {
 
   // updateSingleSurfaceStatus on a plane, SteppingHelper.hpp:31-65 (+
   // PlanarHelper.hpp:28-49); returns 0 unreachable, 1 reachable, 2 on surface
-  KFD_INLINE int surfaceStatus(const DevPlane &pl) {
+  KFD_INLINE int surfaceStatus(int idx) {
     const double nd = (double)navDir;
-    const double d[3] = {nd * dir[0], nd * dir[1], nd * dir[2]};
-    const double nr[3] = {pl.r[2], pl.r[5], pl.r[8]};
-    const double denom = dot3(d, nr);
-    if (denom != 0.0) {
+    double denom, path;
+    if (idx == pathPlane) {
+      denom = pathDenom;
+      path = pathLen;
+    } else {
+      const DevPlane &pl = plane(idx);
+      const double d[3] = {nd * dir[0], nd * dir[1], nd * dir[2]};
+      const double nr[3] = {pl.r[2], pl.r[5], pl.r[8]};
+      denom = dot3(d, nr);
       const double cm[3] = {pl.c[0] - pos[0], pl.c[1] - pos[1], pl.c[2] - pos[2]};
-      const double path = dot3(nr, cm) / denom;
+      path = (denom != 0.0) ? dot3(nr, cm) / denom : 0.0;
+      pathPlane = idx;
+      pathDenom = denom;
+      pathLen = path;
+    }
+    if (denom != 0.0) {
       if (path * path < kOnSurfaceTolerance * kOnSurfaceTolerance) {
         ss.release(kActor);
         return 2;
@@ -664,7 +706,7 @@ struct TrackFit {
     cur = -1;
     if (next != size) {
       const int sp = seqFirst + next;
-      if (surfaceStatus(plane(sp)) == 2) {
+      if (surfaceStatus(sp) == 2) {
         cur = sp;
         ++next;
         if (next != size) ss.release(kActor);
@@ -676,7 +718,7 @@ struct TrackFit {
   KFD_INLINE void navTarget() {
     cur = -1;
     if (next != size) {
-      if (surfaceStatus(plane(seqFirst + next)) == 0) ++next;
+      if (surfaceStatus(seqFirst + next) == 0) ++next;
     } else {
       navBreak = true;  // the perigee target is always set
     }
@@ -794,12 +836,12 @@ struct TrackFit {
     p = (a.mass < nextE) ? sqrt(nextE * nextE - a.mass * a.mass) : 0.0;
     const double mode = (navDir == 1) ? 1.0 : -1.0;
     double v;  // std::max(0, v) of updateVariance
-    v = cov[sidx(2, 2)] + copysign((double)vPhi, mode);
-    cov[sidx(2, 2)] = (0.0 < v) ? v : 0.0;
-    v = cov[sidx(3, 3)] + copysign((double)vTheta, mode);
-    cov[sidx(3, 3)] = (0.0 < v) ? v : 0.0;
-    v = cov[sidx(4, 4)] + copysign((double)vQop, mode);
-    cov[sidx(4, 4)] = (0.0 < v) ? v : 0.0;
+    v = (double)cov[sidx(2, 2)] + copysign((double)vPhi, mode);
+    cov[sidx(2, 2)] = (S)((0.0 < v) ? v : 0.0);
+    v = (double)cov[sidx(3, 3)] + copysign((double)vTheta, mode);
+    cov[sidx(3, 3)] = (S)((0.0 < v) ? v : 0.0);
+    v = (double)cov[sidx(4, 4)] + copysign((double)vQop, mode);
+    cov[sidx(4, 4)] = (S)((0.0 < v) ? v : 0.0);
   }
 
   // -------------------------------------------------------------- filter ----
@@ -864,117 +906,130 @@ struct TrackFit {
     double xf[6];
 #pragma unroll
     for (int r = 0; r < 6; ++r) xf[r] = xp[r] + (K0[r] * r0 + K1[r] * r1);
-    double cf[21];
+    // the update in place on cov: rows 0 and 1 of the result go to t01 first
+    // (every entry reads the old rows 0 and 1), then rows 2..5 overwrite
+    // cov, then rows 0 and 1 are written back
+    double t01[11];
     if constexpr (kMixed) {
-      // B = (I - K H) P, needed at (r, c) with r <= c or c < 2
-      double B[6][6];
+      // B = (I - K H) P: columns 0 and 1 here, the other (r <= c) entries
+      // where they are consumed
+      double B0[6], B1[6];
 #pragma unroll
-      for (int r = 0; r < 6; ++r)
+      for (int r = 0; r < 6; ++r) {
+        B0[r] = cov[sidx(r, 0)] - K0[r] * cov[sidx(0, 0)] - K1[r] * cov[sidx(1, 0)];
+        B1[r] = cov[sidx(r, 1)] - K0[r] * cov[sidx(0, 1)] - K1[r] * cov[sidx(1, 1)];
+      }
+      auto joseph = [&](int r, int c) {
+        const double brc = (c == 0) ? B0[r] : (c == 1) ? B1[r]
+                                    : cov[sidx(r, c)] - K0[r] * cov[sidx(0, c)] - K1[r] * cov[sidx(1, c)];
+        const double kv0 = V00 * K0[c] + V01 * K1[c];
+        const double kv1 = V01 * K0[c] + V11 * K1[c];
+        return (brc - B0[r] * K0[c] - B1[r] * K1[c]) + (K0[r] * kv0 + K1[r] * kv1);
+      };
 #pragma unroll
-        for (int c = 0; c < 6; ++c)
-          if (r <= c || c < 2) B[r][c] = cov[sidx(r, c)] - K0[r] * cov[sidx(0, c)] - K1[r] * cov[sidx(1, c)];
+      for (int c = 0; c < 6; ++c) t01[c] = joseph(0, c);
 #pragma unroll
-      for (int r = 0; r < 6; ++r)
+      for (int c = 1; c < 6; ++c) t01[5 + c] = joseph(1, c);
 #pragma unroll
-        for (int c = r; c < 6; ++c) {
-          const double kv0 = V00 * K0[c] + V01 * K1[c];
-          const double kv1 = V01 * K0[c] + V11 * K1[c];
-          cf[sidx(r, c)] = (B[r][c] - B[r][0] * K0[c] - B[r][1] * K1[c]) + (K0[r] * kv0 + K1[r] * kv1);
-        }
+      for (int r = 2; r < 6; ++r)
+#pragma unroll
+        for (int c = r; c < 6; ++c) cov[sidx(r, c)] = (S)joseph(r, c);
     } else {
+      auto standard = [&](int r, int c) {
+        const double ca = ((r == 0) ? 1.0 : 0.0) - K0[r];
+        const double cb = ((r == 1) ? 1.0 : 0.0) - K1[r];
+        double s = ca * cov[sidx(0, c)] + cb * cov[sidx(1, c)];
+        if (r >= 2) s = s + cov[sidx(r, c)];
+        return s;
+      };
 #pragma unroll
-      for (int r = 0; r < 6; ++r)
+      for (int c = 0; c < 6; ++c) t01[c] = standard(0, c);
 #pragma unroll
-        for (int c = r; c < 6; ++c) {
-          const double ca = ((r == 0) ? 1.0 : 0.0) - K0[r];
-          const double cb = ((r == 1) ? 1.0 : 0.0) - K1[r];
-          double s = ca * cov[sidx(0, c)] + cb * cov[sidx(1, c)];
-          if (r >= 2) s = s + cov[sidx(r, c)];
-          cf[sidx(r, c)] = s;
-        }
+      for (int c = 1; c < 6; ++c) t01[5 + c] = standard(1, c);
+#pragma unroll
+      for (int r = 2; r < 6; ++r)
+#pragma unroll
+        for (int c = r; c < 6; ++c) cov[sidx(r, c)] = (S)standard(r, c);
     }
+#pragma unroll
+    for (int c = 0; c < 6; ++c) cov[sidx(0, c)] = (S)t01[c];
+#pragma unroll
+    for (int c = 1; c < 6; ++c) cov[sidx(1, c)] = (S)t01[5 + c];
     const double chi2 = r0 * (i00 * r0 + i01 * r1) + r1 * (i10 * r0 + i11 * r1);
     chi2sum = chi2sum + chi2;
     correctParams(xf);
 #pragma unroll
     for (int m = 0; m < 6; ++m) *recD(k, KF_SD_XF + m) = xf[m];
 #pragma unroll
-    for (int m = 0; m < 21; ++m) *recS(k, KF_SS_CF + m) = (S)cf[m];
+    for (int m = 0; m < 21; ++m) *recS(k, KF_SS_CF + m) = cov[m];
     if (tsf) {
-      storeBoundF(k, KF_TS_FILTERED, xf, cf);
+      storeBoundF(k, KF_TS_FILTERED, xf, cov);
       *recF(k, KF_TS_CHI2) = chi2;
     }
-    updateFromBound(pl, xf, cf);
+    updateFromBound(pl, xf);
     material(pl);
     // the covariance the next transport starts from differs from the stored
     // filtered one only on these three diagonal entries
-    *recS(k, KF_SS_MD + 0) = (S)cov[sidx(2, 2)];
-    *recS(k, KF_SS_MD + 1) = (S)cov[sidx(3, 3)];
-    *recS(k, KF_SS_MD + 2) = (S)cov[sidx(4, 4)];
+    *recS(k, KF_SS_MD + 0) = cov[sidx(2, 2)];
+    *recS(k, KF_SS_MD + 1) = cov[sidx(3, 3)];
+    *recS(k, KF_SS_MD + 2) = cov[sidx(4, 4)];
     if (tsc) saveCheckpoint(k);
     ++measured;
     return true;
   }
 
   // -------------------------------------------------------------- smoother ----
-  // LU decomposition with partial pivoting of a 6x6 matrix in place, the
-  // method of the reference's host path (Eigen inverse() in calculateInverse,
-  // include/Utilities/Math.hpp:103-112).  Row swaps are predicated selects so
-  // the matrix stays in registers.
-  KFD_INLINE static void lu6(double lu[6][6], int perm[6]) {
+  // X = C^-1 M for a symmetric positive-definite 6x6 C (packed) and six
+  // right-hand sides, in place on M: LDL^T factorisation of the
+  // diagonally scaled matrix S C S (S = diag(C)^-1/2, unit diagonal, so no
+  // pivoting is needed), then forward substitution, scaling by D^-1 and back
+  // substitution per column.  The reference inverts C explicitly (Eigen's
+  // partial-pivoting LU on the host, cofactors on the device,
+  // include/Utilities/Math.hpp:74-112); solving is ~3x cheaper and agrees to
+  // rounding.  A non-positive diagonal entry gives NaNs (caught by the
+  // smoother's NaN check, as the reference's).
+  KFD_INLINE static void spdSolve(const double C[21], double M[6][6]) {
+    double sc[6];
 #pragma unroll
-    for (int k = 0; k < 6; ++k) perm[k] = k;
+    for (int k = 0; k < 6; ++k) sc[k] = 1.0 / sqrt(C[sidx(k, k)]);
+    // W[r][k] = L[r][k] D[k] (r > k), L below the diagonal, Dinv = 1 / D
+    double L[6][6], W[6][6], Dinv[6];
 #pragma unroll
-    for (int k = 0; k < 6; ++k) {
-      int pv = k;
-      double best = fabs(lu[k][k]);
+    for (int j = 0; j < 6; ++j) {
+      double dj = (sc[j] * C[sidx(j, j)]) * sc[j];
 #pragma unroll
-      for (int r = k + 1; r < 6; ++r) {
-        const double v = fabs(lu[r][k]);
-        if (best < v) {
-          best = v;
-          pv = r;
-        }
-      }
+      for (int k = 0; k < j; ++k) dj = dj - W[j][k] * L[j][k];
+      Dinv[j] = 1.0 / dj;
 #pragma unroll
-      for (int r = k + 1; r < 6; ++r) {
-        const bool sw = (r == pv);
+      for (int r = j + 1; r < 6; ++r) {
+        double w = (sc[r] * C[sidx(r, j)]) * sc[j];
 #pragma unroll
-        for (int c = 0; c < 6; ++c) {
-          const double t = lu[k][c];
-          lu[k][c] = sw ? lu[r][c] : lu[k][c];
-          lu[r][c] = sw ? t : lu[r][c];
-        }
-        const int tp = perm[k];
-        perm[k] = sw ? perm[r] : perm[k];
-        perm[r] = sw ? tp : perm[r];
-      }
-#pragma unroll
-      for (int r = k + 1; r < 6; ++r) {
-        lu[r][k] = lu[r][k] / lu[k][k];
-#pragma unroll
-        for (int c = k + 1; c < 6; ++c) lu[r][c] = lu[r][c] - lu[r][k] * lu[k][c];
+        for (int k = 0; k < j; ++k) w = w - W[r][k] * L[j][k];
+        W[r][j] = w;
+        L[r][j] = w * Dinv[j];
       }
     }
-  }
-
-  // Column c of the inverse from the LU factors (forward then back
-  // substitution on the permuted unit vector).
-  KFD_INLINE static void invColumn(const double lu[6][6], const int perm[6], int c, double x[6]) {
-    double y[6];
 #pragma unroll
-    for (int r = 0; r < 6; ++r) {
-      double s = (perm[r] == c) ? 1.0 : 0.0;
+    for (int c = 0; c < 6; ++c) {
+      double y[6];
 #pragma unroll
-      for (int k = 0; k < r; ++k) s = s - lu[r][k] * y[k];
-      y[r] = s;
-    }
+      for (int r = 0; r < 6; ++r) {
+        double v = sc[r] * M[r][c];
 #pragma unroll
-    for (int r = 5; r >= 0; --r) {
-      double s = y[r];
+        for (int k = 0; k < r; ++k) v = v - L[r][k] * y[k];
+        y[r] = v;
+      }
 #pragma unroll
-      for (int k = r + 1; k < 6; ++k) s = s - lu[r][k] * x[k];
-      x[r] = s / lu[r][r];
+      for (int r = 0; r < 6; ++r) y[r] = y[r] * Dinv[r];
+#pragma unroll
+      for (int r = 5; r >= 0; --r) {
+        double v = y[r];
+#pragma unroll
+        for (int k = r + 1; k < 6; ++k) v = v - L[k][r] * y[k];
+        y[r] = v;
+      }
+#pragma unroll
+      for (int r = 0; r < 6; ++r) M[r][c] = sc[r] * y[r];
     }
   }
 
@@ -1028,41 +1083,31 @@ struct TrackFit {
         loadJacobian(k + 1, J);
         predictedCov(k, J, Cp);
         KFD_PHASE_FENCE();
-        double lu[6][6];
-        int perm[6];
+        // G^T = C_p^-1 (J C_f) (C_f, C_p symmetric); row 4 of J is e4 and
+        // column 5 is e5
+        double cf[21];
+        loadFilteredCov(k, cf);
+        double X[6][6];
 #pragma unroll
         for (int r = 0; r < 6; ++r)
 #pragma unroll
-          for (int c = 0; c < 6; ++c) lu[r][c] = Cp[sidx(r, c)];
-        lu6(lu, perm);
-        double A1[6][6];  // C_f J^T
-        {
-          double cf[21];
-          loadFilteredCov(k, cf);
-#pragma unroll
           for (int c = 0; c < 6; ++c) {
+            double s;
+            if (r == 4) {
+              s = cf[sidx(4, c)];
+            } else {
+              s = (double)J[r][0] * cf[sidx(0, c)];
 #pragma unroll
-            for (int r = 0; r < 6; ++r) {
-              double s = cf[sidx(r, 0)] * (double)J[c][0];
-#pragma unroll
-              for (int m = 1; m < 6; ++m) s = s + cf[sidx(r, m)] * (double)J[c][m];
-              A1[r][c] = s;
+              for (int m = 1; m < 5; ++m) s = s + (double)J[r][m] * cf[sidx(m, c)];
+              if (r == 5) s = s + cf[sidx(5, c)];
             }
+            X[r][c] = s;
           }
-        }
-        KFD_PHASE_FENCE();
+        spdSolve(Cp, X);
 #pragma unroll
-        for (int c = 0; c < 6; ++c) {
-          double ic[6];
-          invColumn(lu, perm, c, ic);
+        for (int r = 0; r < 6; ++r)
 #pragma unroll
-          for (int r = 0; r < 6; ++r) {
-            double s = A1[r][0] * ic[0];
-#pragma unroll
-            for (int m = 1; m < 6; ++m) s = s + A1[r][m] * ic[m];
-            G[r][c] = s;
-          }
-        }
+          for (int c = 0; c < 6; ++c) G[r][c] = X[c][r];
       }
       bool nan = false;  // GainMatrixSmoother.hpp:63-66
 #pragma unroll
@@ -1351,8 +1396,11 @@ struct TrackFit {
     smoothed = true;
     double xs[6], cs[21];
     if (smooth(xs, cs)) {
-      updateFromBound(plane(firstPlane), xs, cs);
+      updateFromBound(plane(firstPlane), xs);
+#pragma unroll
+      for (int m = 0; m < 21; ++m) cov[m] = (S)cs[m];
       navDir = -1;
+      moved();
       ss.init(-1.0 * fabs(a.max_step_size));
       pathAcc = 0.0;
     } else {
