@@ -5,6 +5,7 @@
 // two HIP events that time the last fit kernel on its launch stream.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cfloat>
 #include <cmath>
 #include <cstdarg>
@@ -34,6 +35,7 @@ struct kf_ctx {
   void *adj = nullptr;      // chi2-adjoint buffers (step stack, plane adjoints, ...)
   size_t adj_bytes = 0;
   bool has_material = false;  // a plane of the table carries material
+  bool unique_geo = false;    // the planes' geo ids are distinct
   uint64_t budget = KF_DEFAULT_MEMORY_BUDGET;  // per-batch working buffers
   hipEvent_t ev0 = nullptr, evm = nullptr, ev1 = nullptr;
   bool timed = false;
@@ -185,6 +187,10 @@ int kf_set_geometry(kf_ctx *ctx, const kf_plane *planes, int32_t n_planes) {
   ctx->n_planes = n_planes;
   ctx->has_material = false;
   for (const kfd::DevPlane &d : dp) ctx->has_material = ctx->has_material || d.has_material;
+  std::vector<int32_t> ids(n_planes);
+  for (int i = 0; i < n_planes; ++i) ids[i] = planes[i].geo_id;
+  std::sort(ids.begin(), ids.end());
+  ctx->unique_geo = std::adjacent_find(ids.begin(), ids.end()) == ids.end();
   return KF_OK;
 }
 
@@ -277,6 +283,7 @@ int setupArgs(kf_ctx *ctx, const kf_tracks *tr, const kf_options *opt, const cha
   a.hit_cov = tr->hit_cov;
   a.planes = ctx->planes;
   a.n_planes = ctx->n_planes;
+  a.unique_geo = ctx->unique_geo ? 1 : 0;
   a.field = ctx->field;
   a.smoothing = opt->smoothing;
   a.ms = opt->multiple_scattering;

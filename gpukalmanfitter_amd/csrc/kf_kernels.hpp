@@ -54,6 +54,9 @@ struct FitArgs {
   const double *hit_cov;
   const DevPlane *planes;
   int32_t n_planes;
+  // the geo ids of the plane table are distinct (lets the hit search of a
+  // track resume after its last match)
+  int32_t unique_geo;
   DevField field;
   // options
   int32_t smoothing, ms, eloss, max_steps;

@@ -63,6 +63,25 @@ struct TrackFit {
   int firstPlane;  // plane-table index of the first measured state
   bool smoothed, finished, result, smootherNaN;
   int64_t h0, h1;
+  // hit search: while every match so far was the next hit in order and the
+  // plane geo ids are distinct (inOrder), no hit before hitNext can match a
+  // later plane, so the search starts at hitNext, whose data is loaded ahead
+  // (pf*) while the track steps to the next surface
+  int64_t hitNext;
+  bool inOrder, pfValid;
+  int32_t pfGeo;
+  double pfM0, pfM1, pfV00, pfV01, pfV11;
+  KFD_INLINE void prefetchHit() {
+    pfValid = hitNext < h1;
+    if (pfValid) {
+      pfGeo = a.hit_geo_id[hitNext];
+      pfM0 = a.hit_meas[hitNext];
+      pfM1 = a.hit_meas[a.n_hits + hitNext];
+      pfV00 = a.hit_cov[hitNext];
+      pfV01 = a.hit_cov[a.n_hits + hitNext];
+      pfV11 = a.hit_cov[2 * a.n_hits + hitNext];
+    }
+  }
   double chi2sum;
   double lc[3];  // perigee line point
   double fpar[6];
@@ -429,8 +448,10 @@ struct TrackFit {
     crossStep(dk2dT, half_h, Bm, qop, dk3dT);
     crossStep(dk3dT, h, Bl, qop, dk4dT);
 
-    const S h6 = h * (S)(1.0 / 6.0);
-    const S hh6 = h * h6;
+    // divisions: a float 1/6 constant is biased by 3e-8 relative, which
+    // accumulates over the steps of a track
+    const S h6 = h / (S)6.0;
+    const S hh6 = (h * h) / (S)6.0;
     S dFdT[3][3], dGdT[3][3], dFdL[3], dGdL[3];
 #pragma unroll
     for (int r = 0; r < 3; ++r) {
@@ -853,21 +874,39 @@ struct TrackFit {
   // by construction.  The 2x2 inversion and the update are double in both.
   KFD_INLINE bool filter(int sp) {
     const DevPlane &pl = plane(sp);
-    // the track's hit on this plane: first match in hit order, four
-    // independent geo-id loads per round trip
+    // the track's hit on this plane: first match in hit order (SurfaceFinder,
+    // KalmanFitter.hpp:31-38); the loaded-ahead next hit, else a scan with
+    // four independent geo-id loads per round trip
     int64_t hit = -1;
-    for (int64_t h = h0; h < h1 && hit < 0; h += 4) {
-      int32_t g[4];
+    double m0, m1, V00, V01, V11;
+    if (inOrder && pfValid && pfGeo == pl.geo_id) {
+      hit = hitNext;
+      m0 = pfM0;
+      m1 = pfM1;
+      V00 = pfV00;
+      V01 = pfV01;
+      V11 = pfV11;
+    } else {
+      for (int64_t h = inOrder ? hitNext : h0; h < h1 && hit < 0; h += 4) {
+        int32_t g[4];
 #pragma unroll
-      for (int q = 0; q < 4; ++q) g[q] = (h + q < h1) ? a.hit_geo_id[h + q] : 0;
+        for (int q = 0; q < 4; ++q) g[q] = (h + q < h1) ? a.hit_geo_id[h + q] : 0;
 #pragma unroll
-      for (int q = 0; q < 4; ++q)
-        if (hit < 0 && h + q < h1 && g[q] == pl.geo_id) hit = h + q;
+        for (int q = 0; q < 4; ++q)
+          if (hit < 0 && h + q < h1 && g[q] == pl.geo_id) hit = h + q;
+      }
+      if (hit < 0) return true;  // hole
+      m0 = a.hit_meas[hit];
+      m1 = a.hit_meas[a.n_hits + hit];
+      V00 = a.hit_cov[hit];
+      V01 = a.hit_cov[a.n_hits + hit];
+      V11 = a.hit_cov[2 * a.n_hits + hit];
+      inOrder = inOrder && hit == hitNext;
     }
-    if (hit < 0) return true;  // hole
-    const double m0 = a.hit_meas[hit], m1 = a.hit_meas[a.n_hits + hit];
-    const double V00 = a.hit_cov[hit], V01 = a.hit_cov[a.n_hits + hit];
-    const double V11 = a.hit_cov[2 * a.n_hits + hit];
+    if (inOrder) {
+      hitNext = hit + 1;
+      prefetchHit();
+    }
     if (measured >= a.max_states) return false;
     const int k = measured;
     if (k == 0) firstPlane = sp;
@@ -1039,34 +1078,33 @@ struct TrackFit {
     for (int m = 0; m < 21; ++m) c[m] = (double)*recS(k, KF_SS_CF + m);
   }
 
-  // Transport Jacobian of state k (rows 0, 1, 2, 3, 5; row 4 = e4, col 5 = e5).
-  KFD_INLINE void loadJacobian(int k, S J[6][6]) const {
+  // Stored operands of smoother step k: the Jacobian and predicted
+  // parameters of state k + 1, the filtered covariance, post-material
+  // diagonal and filtered parameters of state k.
+  struct SmoothIn {
+    S j[25];
+    S cf[21];
+    S md[3];
+    double xf[6];
+    double xpn[6];
+  };
+  KFD_INLINE void loadSmoothIn(int k, SmoothIn &in) const {
 #pragma unroll
-    for (int r = 0; r < 6; ++r)
+    for (int m = 0; m < 25; ++m) in.j[m] = *recS(k + 1, KF_SS_J + m);
 #pragma unroll
-      for (int c = 0; c < 6; ++c) {
-        if (r == 4 || c == 5)
-          J[r][c] = (r == c) ? (S)1 : (S)0;
-        else
-          J[r][c] = *recS(k, KF_SS_J + jidx(r, c));
-      }
-  }
-
-  // Predicted covariance of state k + 1: the forward pass's transport of the
-  // post-material covariance of state k, recomputed from the same operands.
-  KFD_INLINE void predictedCov(int k, const S J[6][6], double cp[21]) const {
-    S cm[21];
+    for (int m = 0; m < 21; ++m) in.cf[m] = *recS(k, KF_SS_CF + m);
 #pragma unroll
-    for (int m = 0; m < 21; ++m) cm[m] = *recS(k, KF_SS_CF + m);
-    cm[sidx(2, 2)] = *recS(k, KF_SS_MD + 0);
-    cm[sidx(3, 3)] = *recS(k, KF_SS_MD + 1);
-    cm[sidx(4, 4)] = *recS(k, KF_SS_MD + 2);
-    sandwich(J, cm, cp);
+    for (int m = 0; m < 3; ++m) in.md[m] = *recS(k, KF_SS_MD + m);
+#pragma unroll
+    for (int m = 0; m < 6; ++m) in.xf[m] = *recD(k, KF_SD_XF + m);
+#pragma unroll
+    for (int m = 0; m < 6; ++m) in.xpn[m] = *recD(k + 1, KF_SD_XP + m);
   }
 
   // GainMatrixSmoother (GainMatrixSmoother.hpp:21-94):
   // G = (C_f J^T) C_p^-1, x_s = x_f + G (x_s' - x_p'), C_s = C_f - G (C_p - C_s') G^T,
-  // in double in both modes.
+  // in double in both modes.  The operands of step k - 1 are loaded while
+  // step k computes.
   KFD_INLINE bool smooth(double xs[6], double cs[21]) {
     const int n = measured;
     double xsn[6], csn[21];
@@ -1075,18 +1113,36 @@ struct TrackFit {
     for (int m = 0; m < 6; ++m) xsn[m] = *recD(n - 1, KF_SD_XF + m);
     loadFilteredCov(n - 1, csn);
     if (tsf) storeBoundF(n - 1, KF_TS_SMOOTHED, xsn, csn);
+    SmoothIn cur;
+    if (n >= 2) loadSmoothIn(n - 2, cur);
     for (int k = n - 2; k >= 0; --k) {
-      double G[6][6];
+      SmoothIn nxt;
+      loadSmoothIn((k > 0) ? k - 1 : 0, nxt);
+      S J[6][6];
+#pragma unroll
+      for (int r = 0; r < 6; ++r)
+#pragma unroll
+        for (int c = 0; c < 6; ++c)
+          J[r][c] = (r == 4 || c == 5) ? ((r == c) ? (S)1 : (S)0) : cur.j[jidx(r, c)];
+      // predicted covariance of state k + 1: the forward pass's transport of
+      // the post-material covariance of state k, from the same operands
       double Cp[21];
       {
-        S J[6][6];
-        loadJacobian(k + 1, J);
-        predictedCov(k, J, Cp);
-        KFD_PHASE_FENCE();
-        // G^T = C_p^-1 (J C_f) (C_f, C_p symmetric); row 4 of J is e4 and
-        // column 5 is e5
-        double cf[21];
-        loadFilteredCov(k, cf);
+        S cm[21];
+#pragma unroll
+        for (int m = 0; m < 21; ++m) cm[m] = cur.cf[m];
+        cm[sidx(2, 2)] = cur.md[0];
+        cm[sidx(3, 3)] = cur.md[1];
+        cm[sidx(4, 4)] = cur.md[2];
+        sandwich(J, cm, Cp);
+      }
+      double cf[21];
+#pragma unroll
+      for (int m = 0; m < 21; ++m) cf[m] = (double)cur.cf[m];
+      // G^T = C_p^-1 (J C_f) (C_f, C_p symmetric); row 4 of J is e4 and
+      // column 5 is e5
+      double G[6][6];
+      {
         double X[6][6];
 #pragma unroll
         for (int r = 0; r < 6; ++r)
@@ -1118,23 +1174,19 @@ struct TrackFit {
         smootherNaN = true;
         return false;
       }
-      KFD_PHASE_FENCE();
       double dx[6];
 #pragma unroll
-      for (int r = 0; r < 6; ++r) dx[r] = xsn[r] - *recD(k + 1, KF_SD_XP + r);
+      for (int r = 0; r < 6; ++r) dx[r] = xsn[r] - cur.xpn[r];
 #pragma unroll
       for (int r = 0; r < 6; ++r) {
         double s = G[r][0] * dx[0];
 #pragma unroll
         for (int m = 1; m < 6; ++m) s = s + G[r][m] * dx[m];
-        xsn[r] = *recD(k, KF_SD_XF + r) + s;
+        xsn[r] = cur.xf[r] + s;
       }
-      KFD_PHASE_FENCE();
       double Dm[21];
 #pragma unroll
       for (int m = 0; m < 21; ++m) Dm[m] = Cp[m] - csn[m];
-      double cf[21];
-      loadFilteredCov(k, cf);
 #pragma unroll
       for (int r = 0; r < 6; ++r) {
         double B1[6];  // row r of G (C_p - C_s')
@@ -1155,6 +1207,7 @@ struct TrackFit {
       }
       correctParams(xsn);
       if (tsf) storeBoundF(k, KF_TS_SMOOTHED, xsn, csn);
+      cur = nxt;
     }
 #pragma unroll
     for (int m = 0; m < 6; ++m) xs[m] = xsn[m];
@@ -1323,6 +1376,9 @@ struct TrackFit {
   KFD_INLINE void loadTrack() {
     h0 = a.hit_offset[i];
     h1 = a.hit_offset[i + 1];
+    hitNext = h0;
+    inOrder = a.unique_geo != 0;
+    pfValid = false;
     seqFirst = a.seq_first[i];
     size = a.seq_count[i];
 #pragma unroll
@@ -1358,6 +1414,7 @@ struct TrackFit {
     const bool bad = size <= 0 || seqFirst < 0 || seqFirst + size > a.n_planes || h0 < 0 ||
                      h1 <= h0 || h1 > a.n_hits || pv[4] == 0.0 || a.max_states < 1;
     if (!bad) {
+      prefetchHit();
       start(pv);
       navStatus();
       if (actor()) {

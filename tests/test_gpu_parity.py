@@ -163,6 +163,58 @@ def test_edge_cases(torch_cuda):
     assert np.array_equal(o.status, st[[0, 2, 3, 4, 5]])
 
 
+def _with_hits(b, order):
+    """b with the hits of every track replaced by order[t] (hit indices)."""
+    import copy
+    nb = copy.copy(b)
+    idx = np.concatenate([np.asarray(o, dtype=np.int64) for o in order])
+    nb.hit_offset = np.concatenate([[0], np.cumsum([len(o) for o in order])]).astype(np.int64)
+    nb.hit_geo_id = np.ascontiguousarray(b.hit_geo_id[idx])
+    nb.hit_meas = np.ascontiguousarray(b.hit_meas[:, idx])
+    nb.hit_cov = np.ascontiguousarray(b.hit_cov[:, idx])
+    return nb
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_hit_order_and_duplicate_hits(torch_cuda, precision):
+    """The hit of a plane is the track's first hit with that geo id, in hit
+    order.  Shuffled hit lists fit bitwise as the ordered ones (the search
+    leaves its in-order fast path); a copy of a track's fourth hit with a
+    moved measurement, placed before its second hit, is the one used; the
+    oracle agrees on every track."""
+    b = G.generate(3, n_tracks=3000)
+    rng = np.random.default_rng(3)
+    ranges = [np.arange(b.hit_offset[t], b.hit_offset[t + 1]) for t in range(b.n_tracks)]
+    shuffled = [rng.permutation(r) if t % 2 == 0 else r for t, r in enumerate(ranges)]
+    dup = []
+    for t, r in enumerate(ranges):
+        if t % 3 == 0 and len(r) >= 4:
+            dup.append(np.concatenate([r[:1], [r[3]], r[1:]]))
+        else:
+            dup.append(r)
+    bd = _with_hits(b, dup)
+    moved = np.zeros(bd.n_hits, dtype=bool)
+    for t in range(b.n_tracks):
+        if t % 3 == 0 and len(ranges[t]) >= 4:
+            moved[bd.hit_offset[t] + 1] = True
+    bd.hit_meas = bd.hit_meas.copy()
+    bd.hit_meas[0, moved] += 0.1  # 2 sigma: changes the fit of those tracks
+    opts = dict(W, precision=precision)
+    ref = run_gpu(b, **opts)
+    got = run_gpu(_with_hits(b, shuffled), **opts)
+    for k in ("params", "cov", "chi2", "status", "n_measured"):
+        assert np.array_equal(got[k], ref[k]), k
+    g = run_gpu(bd, **opts)
+    o, _, _, _ = oracle.fit(bd, c_options(**W))
+    sel = [t for t in range(b.n_tracks) if t % 3 == 0 and len(ranges[t]) >= 4]
+    assert np.abs(g["chi2"][sel] - ref["chi2"][sel]).max() > 1.0  # the copy was used
+    if precision == "fp64":
+        check_parity(g, o)
+    else:
+        pd, cd, _ = compare_fit(g, o)
+        assert pd <= TOL_MIXED_PARAMS and cd <= TOL_MIXED_COV, (pd, cd)
+
+
 def test_empty_batch_and_single_track(torch_cuda):
     b = G.generate(1, n_tracks=1)
     g = run_gpu(b, **W)
