@@ -13,14 +13,15 @@
 // count); larger tables are read from global memory.
 #define KF_LDS_PLANES 256
 
-// per-state store fields (see FitArgs)
+// per-state store elements (see FitArgs)
 #define KF_SD_XP 0   // predicted parameters (6)
 #define KF_SD_XF 6   // filtered parameters (6)
 #define KF_SD_N 12
-#define KF_SS_CF 0   // filtered covariance (21, packed upper triangle)
-#define KF_SS_MD 21  // covariance (2,2), (3,3), (4,4) after the material update
-#define KF_SS_J 24   // Jacobian rows 0, 1, 2, 3, 5 x columns 0..4 (25)
-#define KF_SS_N 49
+#define KF_SS_J 0    // Jacobian rows 0, 1, 2, 3, 5 x columns 0..4 (25, padded to 28)
+#define KF_SS_CF 28  // filtered covariance (21, packed upper triangle)
+#define KF_SS_MD 49  // covariance (2,2), (3,3), (4,4) after the material update
+#define KF_SS_N 52
+#define KF_SS_USED 49  // elements that carry data (the rest is padding)
 // per-surface checkpoints of the chi2 adjoint (state after the update and
 // material at a measured surface; see FitArgs::tsc)
 #define KF_CK_POS 0    // 3
@@ -72,16 +73,21 @@ struct FitArgs {
   int32_t *out_ndf;
   int32_t *out_nmeas;
   uint32_t *out_status;
-  // Per-state store of the smoother, chunked by 64 tracks: element f of
-  // state k of track i sits at
-  //   base[((i / 64) * max_states * F + k * F + f) * 64 + i % 64].
+  // Per-state store of the smoother, chunked by 64 track positions and kept
+  // in 16-byte groups per lane (W = 16 / sizeof(element) elements): element
+  // f of state k of position i sits at
+  //   base[(((i / 64) * max_states * F + k * F + f - f % W) * 64 + (i % 64) * W) + f % W],
+  // so one 16-byte vector access per lane moves W elements and a wave's
+  // access is 1 KB contiguous.
   //  tsd  double, F = KF_SD_N: predicted and filtered parameters;
   //  tss  the transport scalar S (double in FP64 mode, float in mixed mode),
-  //       F = KF_SS_N: filtered covariance, the three variances after the
-  //       material update, and the 25 free entries of the bound-to-bound
-  //       transport Jacobian (row 4 is e4, column 5 is e5 on every surface);
+  //       F = KF_SS_N: the 25 free entries of the bound-to-bound transport
+  //       Jacobian (row 4 is e4, column 5 is e5 on every surface), the
+  //       filtered covariance and the three variances after the material
+  //       update;
   //  tsf  double, F = KF_TS_STRIDE: the public track-state record, written
-  //       only when the caller asks for track states (else NULL).
+  //       only when the caller asks for track states (else NULL), one
+  //       element per lane: base[((i / 64) * max_states * F + k * F + f) * 64 + i % 64].
   // The predicted covariance is not stored: the smoother recomputes it from
   // the stored Jacobian and post-material covariance with the same operation
   // sequence, so it is bit-identical to the forward pass's.

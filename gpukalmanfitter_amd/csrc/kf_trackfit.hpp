@@ -131,9 +131,54 @@ struct TrackFit {
   KFD_INLINE static uint32_t slot(int k, int F, int f, uint32_t ln) {
     return (uint32_t)(((uint32_t)k * (uint32_t)F + (uint32_t)f) * 64u + ln);
   }
-  KFD_INLINE double *recD(int k, int f) const { return tsd + slot(k, KF_SD_N, f, lane); }
-  KFD_INLINE S *recS(int k, int f) const { return tss + slot(k, KF_SS_N, f, lane); }
   KFD_INLINE double *recF(int k, int f) const { return tsf + slot(k, KF_TS_STRIDE, f, lane); }
+  // The smoother stores tsd / tss in 16-byte groups per lane: the group of
+  // element f (f a multiple of W = 16 / sizeof(element)) of state k at
+  // base[(k * F + f) * 64 + lane * W].
+  static constexpr int kW = 16 / (int)sizeof(S);
+  KFD_INLINE static uint32_t gslot(int k, int F, int f, int W, uint32_t ln) {
+    return (uint32_t)(((uint32_t)k * (uint32_t)F + (uint32_t)f) * 64u + ln * (uint32_t)W);
+  }
+  KFD_INLINE double *grpD(int k, int f) const { return tsd + gslot(k, KF_SD_N, f, 2, lane); }
+  KFD_INLINE S *grpS(int k, int f) const { return tss + gslot(k, KF_SS_N, f, kW, lane); }
+  KFD_INLINE static void st16(float *p, const float *v) {
+    *(float4 *)p = make_float4(v[0], v[1], v[2], v[3]);
+  }
+  KFD_INLINE static void st16(double *p, const double *v) {
+    *(double2 *)p = make_double2(v[0], v[1]);
+  }
+  KFD_INLINE static void ld16(const float *p, float *v) {
+    const float4 x = *(const float4 *)p;
+    v[0] = x.x;
+    v[1] = x.y;
+    v[2] = x.z;
+    v[3] = x.w;
+  }
+  KFD_INLINE static void ld16(const double *p, double *v) {
+    const double2 x = *(const double2 *)p;
+    v[0] = x.x;
+    v[1] = x.y;
+  }
+  // six parameters at tsd element f (0 or 6) of state k
+  KFD_INLINE void storePar(int k, int f, const double v[6]) const {
+#pragma unroll
+    for (int g = 0; g < 6; g += 2) st16(grpD(k, f + g), v + g);
+  }
+  KFD_INLINE void loadPar(int k, int f, double v[6]) const {
+#pragma unroll
+    for (int g = 0; g < 6; g += 2) ld16(grpD(k, f + g), v + g);
+  }
+  // n elements (a multiple of kW) at tss element f of state k
+  template <int N>
+  KFD_INLINE void storeS(int k, int f, const S v[N]) const {
+#pragma unroll
+    for (int g = 0; g < N; g += kW) st16(grpS(k, f + g), v + g);
+  }
+  template <int N>
+  KFD_INLINE void loadS(int k, int f, S v[N]) const {
+#pragma unroll
+    for (int g = 0; g < N; g += kW) ld16(grpS(k, f + g), v + g);
+  }
   KFD_INLINE double *recC(int k, int f) const { return tsc + slot(k, KF_CK_N, f, lane); }
 
   // Checkpoint k: the state right after the update and material of measured
@@ -914,13 +959,18 @@ struct TrackFit {
     S jb[6][6];
     boundStatePlane(pl, xp, jb);
     // predicted parameters and the transport Jacobian to the store
+    storePar(k, KF_SD_XP, xp);
+    {
+      S jv[KF_SS_CF - KF_SS_J];
 #pragma unroll
-    for (int m = 0; m < 6; ++m) *recD(k, KF_SD_XP + m) = xp[m];
+      for (int r = 0; r < 6; ++r)
 #pragma unroll
-    for (int r = 0; r < 6; ++r)
+        for (int c = 0; c < 5; ++c)
+          if (r != 4) jv[jidx(r, c)] = jb[r][c];
 #pragma unroll
-      for (int c = 0; c < 5; ++c)
-        if (r != 4) *recS(k, KF_SS_J + jidx(r, c)) = jb[r][c];
+      for (int m = 25; m < KF_SS_CF - KF_SS_J; ++m) jv[m] = (S)0;
+      storeS<KF_SS_CF - KF_SS_J>(k, KF_SS_J, jv);
+    }
     if (tsf) {
       storeBoundF(k, KF_TS_PREDICTED, xp, cov);
 #pragma unroll
@@ -997,21 +1047,23 @@ struct TrackFit {
     const double chi2 = r0 * (i00 * r0 + i01 * r1) + r1 * (i10 * r0 + i11 * r1);
     chi2sum = chi2sum + chi2;
     correctParams(xf);
-#pragma unroll
-    for (int m = 0; m < 6; ++m) *recD(k, KF_SD_XF + m) = xf[m];
-#pragma unroll
-    for (int m = 0; m < 21; ++m) *recS(k, KF_SS_CF + m) = cov[m];
+    storePar(k, KF_SD_XF, xf);
     if (tsf) {
       storeBoundF(k, KF_TS_FILTERED, xf, cov);
       *recF(k, KF_TS_CHI2) = chi2;
     }
+    // the filtered covariance and, after the material update, the three
+    // diagonal entries in which the covariance the next transport starts
+    // from differs from it
+    S cm[KF_SS_N - KF_SS_CF];
+#pragma unroll
+    for (int m = 0; m < 21; ++m) cm[m] = cov[m];
     updateFromBound(pl, xf);
     material(pl);
-    // the covariance the next transport starts from differs from the stored
-    // filtered one only on these three diagonal entries
-    *recS(k, KF_SS_MD + 0) = cov[sidx(2, 2)];
-    *recS(k, KF_SS_MD + 1) = cov[sidx(3, 3)];
-    *recS(k, KF_SS_MD + 2) = cov[sidx(4, 4)];
+    cm[KF_SS_MD - KF_SS_CF + 0] = cov[sidx(2, 2)];
+    cm[KF_SS_MD - KF_SS_CF + 1] = cov[sidx(3, 3)];
+    cm[KF_SS_MD - KF_SS_CF + 2] = cov[sidx(4, 4)];
+    storeS<KF_SS_N - KF_SS_CF>(k, KF_SS_CF, cm);
     if (tsc) saveCheckpoint(k);
     ++measured;
     return true;
@@ -1074,8 +1126,10 @@ struct TrackFit {
 
   // Filtered covariance of state k, in double.
   KFD_INLINE void loadFilteredCov(int k, double c[21]) const {
+    S cm[KF_SS_N - KF_SS_CF];
+    loadS<KF_SS_N - KF_SS_CF>(k, KF_SS_CF, cm);
 #pragma unroll
-    for (int m = 0; m < 21; ++m) c[m] = (double)*recS(k, KF_SS_CF + m);
+    for (int m = 0; m < 21; ++m) c[m] = (double)cm[m];
   }
 
   // Stored operands of smoother step k: the Jacobian and predicted
@@ -1089,16 +1143,17 @@ struct TrackFit {
     double xpn[6];
   };
   KFD_INLINE void loadSmoothIn(int k, SmoothIn &in) const {
+    S jv[KF_SS_CF - KF_SS_J], cm[KF_SS_N - KF_SS_CF];
+    loadS<KF_SS_CF - KF_SS_J>(k + 1, KF_SS_J, jv);
+    loadS<KF_SS_N - KF_SS_CF>(k, KF_SS_CF, cm);
 #pragma unroll
-    for (int m = 0; m < 25; ++m) in.j[m] = *recS(k + 1, KF_SS_J + m);
+    for (int m = 0; m < 25; ++m) in.j[m] = jv[m];
 #pragma unroll
-    for (int m = 0; m < 21; ++m) in.cf[m] = *recS(k, KF_SS_CF + m);
+    for (int m = 0; m < 21; ++m) in.cf[m] = cm[m];
 #pragma unroll
-    for (int m = 0; m < 3; ++m) in.md[m] = *recS(k, KF_SS_MD + m);
-#pragma unroll
-    for (int m = 0; m < 6; ++m) in.xf[m] = *recD(k, KF_SD_XF + m);
-#pragma unroll
-    for (int m = 0; m < 6; ++m) in.xpn[m] = *recD(k + 1, KF_SD_XP + m);
+    for (int m = 0; m < 3; ++m) in.md[m] = cm[KF_SS_MD - KF_SS_CF + m];
+    loadPar(k, KF_SD_XF, in.xf);
+    loadPar(k + 1, KF_SD_XP, in.xpn);
   }
 
   // GainMatrixSmoother (GainMatrixSmoother.hpp:21-94):
@@ -1109,8 +1164,7 @@ struct TrackFit {
     const int n = measured;
     double xsn[6], csn[21];
     // last state: smoothed = filtered
-#pragma unroll
-    for (int m = 0; m < 6; ++m) xsn[m] = *recD(n - 1, KF_SD_XF + m);
+    loadPar(n - 1, KF_SD_XF, xsn);
     loadFilteredCov(n - 1, csn);
     if (tsf) storeBoundF(n - 1, KF_TS_SMOOTHED, xsn, csn);
     SmoothIn cur;
