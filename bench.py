@@ -62,8 +62,8 @@ HBM_PEAK_GBS = 8000.0
 # calibration (profiles/kf_fit_config2_1M_tracks.md).
 TRAFFIC_TRACKS = 1048576
 TRAFFIC_BYTES_PER_LAUNCH = {
-    ("fp64", "forward"): 4.857e9 + 5.411e9, ("fp64", "backward"): 5.565e9 + 0.231e9,
-    ("mixed", "forward"): 4.640e9 + 3.355e9, ("mixed", "backward"): 3.230e9 + 0.231e9,
+    ("fp64", "forward"): 4.786e9 + 5.662e9, ("fp64", "backward"): 5.341e9 + 0.231e9,
+    ("mixed", "forward"): 4.595e9 + 3.481e9, ("mixed", "backward"): 3.344e9 + 0.231e9,
 }
 PARAM_NAMES = ("loc0", "loc1", "phi", "theta", "qop", "t")
 
@@ -168,8 +168,8 @@ def launch_figures(prec, phase, n, ms):
     traffic = TRAFFIC_BYTES_PER_LAUNCH.get((prec, phase))
     if traffic is not None:
         traffic *= n / TRAFFIC_TRACKS
-    return {"kernel": "kf_fit_kernel<%s, %s>" % ("float" if mixed else "double",
-                                                "true" if phase == "forward" else "false"),
+    return {"kernel": "kf_fit_kernel<%s, %s, true, false>" % ("float" if mixed else "double",
+                                                            "true" if phase == "forward" else "false"),
             "kernel_ms": ms, "bytes_per_track": b, "flops_per_track": f,
             "fp32_flops_per_track": g, "intensity_flop_per_byte": f / b,
             "hbm_gb_per_s": b * n / t / 1e9, "hbm_frac": b * n / t / 1e9 / HBM_PEAK_GBS,
