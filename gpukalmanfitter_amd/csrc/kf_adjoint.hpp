@@ -77,64 +77,6 @@ KFD_INLINE void crsVJP(const double a[3], const double b[3], const double cb[3],
 #pragma unroll
   for (int k = 0; k < 3; ++k) bb[k] += t[k];
 }
-// M(B) v = v x B, as a matrix
-KFD_INLINE void crossMat(const double B[3], double M[3][3]) {
-  M[0][0] = 0.0;
-  M[0][1] = B[2];
-  M[0][2] = -B[1];
-  M[1][0] = -B[2];
-  M[1][1] = 0.0;
-  M[1][2] = B[0];
-  M[2][0] = B[1];
-  M[2][1] = -B[0];
-  M[2][2] = 0.0;
-}
-// adjoint of B from the adjoint of M(B)
-KFD_INLINE void crossMatVJP(const double Mb[3][3], double Bb[3]) {
-  Bb[0] += Mb[1][2] - Mb[2][1];
-  Bb[1] += Mb[2][0] - Mb[0][2];
-  Bb[2] += Mb[0][1] - Mb[1][0];
-}
-
-// T = qop M X, X = I + w Tp.  Tb in; accumulates Bb, qb, Tpb, hb (dX/dh = dwdh Tp).
-KFD_INLINE void chainVJP(const double M[3][3], const double X[3][3], const double T[3][3],
-                         double qop, const double Tb[3][3], double Bb[3], double &qb,
-                         double Tpb[3][3], double w, const double Tp[3][3], double &hb,
-                         double dwdh) {
-  double Mb[3][3], Xb[3][3];
-#pragma unroll
-  for (int r = 0; r < 3; ++r)
-#pragma unroll
-    for (int q = 0; q < 3; ++q) {
-      double sM = 0.0, sX = 0.0;
-#pragma unroll
-      for (int m = 0; m < 3; ++m) {
-        sM += Tb[r][m] * X[q][m];  // (Tb X^T)[r][q]
-        sX += M[m][r] * Tb[m][q];  // (M^T Tb)[r][q]
-      }
-      Mb[r][q] = qop * sM;
-      Xb[r][q] = qop * sX;
-    }
-  // qop-derivative: T = qop (M X) -> qb += <Tb, M X>
-#pragma unroll
-  for (int r = 0; r < 3; ++r)
-#pragma unroll
-    for (int q = 0; q < 3; ++q) {
-      double s = 0.0;
-#pragma unroll
-      for (int m = 0; m < 3; ++m) s += M[r][m] * X[m][q];
-      qb += Tb[r][q] * s;
-    }
-  crossMatVJP(Mb, Bb);
-#pragma unroll
-  for (int r = 0; r < 3; ++r)
-#pragma unroll
-    for (int q = 0; q < 3; ++q) {
-      Tpb[r][q] += w * Xb[r][q];
-      hb += dwdh * Xb[r][q] * Tp[r][q];
-    }
-}
-
 // B = s F(x): sb += Bb . F(x) = Bb . B / s; xb += (dB/dx)^T Bb on a grid.
 KFD_INLINE void fieldVJP(const DevField &fld, const double x[3], const double B[3],
                          const double Bb[3], double xb[3], double &sb) {
@@ -155,9 +97,6 @@ struct StepFwd {
   double t2[3], t3[3], t4[3];
   double p1[3], p2[3];
   double L1[3], L2[3], L3[3], L4[3];
-  double T1[3][3], T2[3][3], T3[3][3], T4[3][3];
-  double X2[3][3], X3[3][3], X4[3][3];
-  double dFdT[3][3], dGdT[3][3], dFdL[3], dGdL[3];
   double du[3], dun;
 
   KFD_INLINE void run(const DevField &fld, const double pos[3], const double d[3], double h,
@@ -191,7 +130,8 @@ struct StepFwd {
 #pragma unroll
     for (int k = 0; k < 3; ++k) du[k] = d[k] + (h / 6.0) * (k1[k] + 2.0 * (k2[k] + k3[k]) + k4[k]);
     dun = sqrt(du[0] * du[0] + du[1] * du[1] + du[2] * du[2]);
-    // transport matrix blocks (EigenStepper.ipp:50-124)
+    // q/p column of the transport (EigenStepper.ipp:50-124): the dk/dlambda
+    // vectors
     double c[3];
     crs(d, B1, L1);
     crs(t2, Bm, L2);
@@ -206,47 +146,101 @@ struct StepFwd {
     crs(L3, Bl, c);
 #pragma unroll
     for (int k = 0; k < 3; ++k) L4[k] += (qop * h) * c[k];
-    double M1[3][3], Mm[3][3], Ml[3][3];
-    crossMat(B1, M1);
-    crossMat(Bm, Mm);
-    crossMat(Bl, Ml);
-#pragma unroll
-    for (int r = 0; r < 3; ++r)
-#pragma unroll
-      for (int q = 0; q < 3; ++q) T1[r][q] = qop * M1[r][q];
-    chain(Mm, T1, hh, qop, X2, T2);
-    chain(Mm, T2, hh, qop, X3, T3);
-    chain(Ml, T3, h, qop, X4, T4);
-#pragma unroll
-    for (int r = 0; r < 3; ++r) {
-#pragma unroll
-      for (int q = 0; q < 3; ++q) {
-        const double id = (r == q) ? 1.0 : 0.0;
-        dFdT[r][q] = h * (id + (h / 6.0) * (T1[r][q] + T2[r][q] + T3[r][q]));
-        dGdT[r][q] = id + (h / 6.0) * (T1[r][q] + 2.0 * (T2[r][q] + T3[r][q]) + T4[r][q]);
-      }
-      dFdL[r] = (h * h / 6.0) * (L1[r] + L2[r] + L3[r]);
-      dGdL[r] = (h / 6.0) * (L1[r] + 2.0 * (L2[r] + L3[r]) + L4[r]);
-    }
-  }
-  // X = I + w Tprev, T = qop M X
-  KFD_INLINE static void chain(const double M[3][3], const double Tp[3][3], double w, double qop,
-                               double X[3][3], double T[3][3]) {
-#pragma unroll
-    for (int r = 0; r < 3; ++r)
-#pragma unroll
-      for (int q = 0; q < 3; ++q) X[r][q] = ((r == q) ? 1.0 : 0.0) + w * Tp[r][q];
-#pragma unroll
-    for (int r = 0; r < 3; ++r)
-#pragma unroll
-      for (int q = 0; q < 3; ++q) {
-        double s = 0.0;
-#pragma unroll
-        for (int m = 0; m < 3; ++m) s += M[r][m] * X[m][q];
-        T[r][q] = qop * s;
-      }
   }
 };
+
+// The direction block of the transport applied to one column g of the
+// Jacobian's direction rows: v1 = qop (g x B1), v2 = qop ((g + hh v1) x Bm),
+// v3 = qop ((g + hh v2) x Bm), v4 = qop ((g + h v3) x Bl); then
+// dFdT g = h g + h^2/6 (v1 + v2 + v3), dGdT g = g + h/6 (v1 + 2 v2 + 2 v3 + v4)
+// (the matrices of EigenStepper.ipp:50-124 applied to a vector).  This is the
+// backward of that map: aF, aG are the adjoints of dFdT g and dGdT g; it
+// accumulates gb, the field adjoints, hb and qb.
+KFD_INLINE void dirColumnVJP(const StepFwd &F, double h, double qop, const double g[3],
+                             const double aF[3], const double aG[3], double gb[3], double B1b[3],
+                             double Bmb[3], double Blb[3], double &hb, double &qb) {
+  const double hh = 0.5 * h;
+  // x_i = u_i x B (u_1 = g), v_i = qop x_i
+  double u2[3], u3[3], u4[3], x1[3], x2[3], x3[3], x4[3], v1[3], v2[3], v3[3], v4[3];
+  crs(g, F.B1, x1);
+#pragma unroll
+  for (int k = 0; k < 3; ++k) v1[k] = qop * x1[k];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) u2[k] = g[k] + hh * v1[k];
+  crs(u2, F.Bm, x2);
+#pragma unroll
+  for (int k = 0; k < 3; ++k) v2[k] = qop * x2[k];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) u3[k] = g[k] + hh * v2[k];
+  crs(u3, F.Bm, x3);
+#pragma unroll
+  for (int k = 0; k < 3; ++k) v3[k] = qop * x3[k];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) u4[k] = g[k] + h * v3[k];
+  crs(u4, F.Bl, x4);
+#pragma unroll
+  for (int k = 0; k < 3; ++k) v4[k] = qop * x4[k];
+  double v1b[3], v2b[3], v3b[3], v4b[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    gb[k] += h * aF[k] + aG[k];
+    hb += aF[k] * (g[k] + (h / 3.0) * (v1[k] + v2[k] + v3[k])) +
+          aG[k] * (v1[k] + 2.0 * (v2[k] + v3[k]) + v4[k]) / 6.0;
+    const double a = (h * h / 6.0) * aF[k], b = (h / 6.0) * aG[k];
+    v1b[k] = a + b;
+    v2b[k] = a + 2.0 * b;
+    v3b[k] = a + 2.0 * b;
+    v4b[k] = b;
+  }
+  double w[3], ub[3];
+  // v4 = qop (u4 x Bl), u4 = g + h v3
+  qb += v4b[0] * x4[0] + v4b[1] * x4[1] + v4b[2] * x4[2];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    w[k] = qop * v4b[k];
+    ub[k] = 0.0;
+  }
+  crsVJP(u4, F.Bl, w, ub, Blb);
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    gb[k] += ub[k];
+    v3b[k] += h * ub[k];
+    hb += ub[k] * v3[k];
+  }
+  // v3 = qop (u3 x Bm), u3 = g + hh v2
+  qb += v3b[0] * x3[0] + v3b[1] * x3[1] + v3b[2] * x3[2];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    w[k] = qop * v3b[k];
+    ub[k] = 0.0;
+  }
+  crsVJP(u3, F.Bm, w, ub, Bmb);
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    gb[k] += ub[k];
+    v2b[k] += hh * ub[k];
+    hb += 0.5 * ub[k] * v2[k];
+  }
+  // v2 = qop (u2 x Bm), u2 = g + hh v1
+  qb += v2b[0] * x2[0] + v2b[1] * x2[1] + v2b[2] * x2[2];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    w[k] = qop * v2b[k];
+    ub[k] = 0.0;
+  }
+  crsVJP(u2, F.Bm, w, ub, Bmb);
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    gb[k] += ub[k];
+    v1b[k] += hh * ub[k];
+    hb += 0.5 * ub[k] * v1[k];
+  }
+  // v1 = qop (g x B1)
+  qb += v1b[0] * x1[0] + v1b[1] * x1[1] + v1b[2] * x1[2];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) w[k] = qop * v1b[k];
+  crsVJP(g, F.B1, w, gb, B1b);
+}
 
 // Backward of one RK step.  In: the adjoints of the step's outputs (position
 // pb, direction db, Jacobian Ab/Gb, and k4b for the propagation derivative of
@@ -262,48 +256,26 @@ KFD_INLINE void stepVJP(const DevField &fld, const StepFwd &F, const double pos[
   double t2b[3] = {0, 0, 0}, t3b[3] = {0, 0, 0}, t4b[3] = {0, 0, 0};
   double B1b[3] = {0, 0, 0}, Bmb[3] = {0, 0, 0}, Blb[3] = {0, 0, 0};
   double dInb[3] = {0, 0, 0}, posInb[3] = {pb[0], pb[1], pb[2]};
-  // ---- J <- D J:  A[:,2+c] += dFdT G[:,c] (+ dFdL for c = 2), G[:,c] = dGdT G[:,c] (+ dGdL)
-  double dFdTb[3][3], dGdTb[3][3], dFdLb[3], dGdLb[3], GinB[3][3];
+  // ---- J <- D J:  A[:,2+c] += dFdT G[:,c] (+ dFdL for c = 2), G[:,c] = dGdT G[:,c] (+ dGdL),
+  // column by column
+  double GinB[3][3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const double g[3] = {Gin[0][c], Gin[1][c], Gin[2][c]};
+    const double aF[3] = {Ab[0][2 + c], Ab[1][2 + c], Ab[2][2 + c]};
+    const double aG[3] = {Gb[0][c], Gb[1][c], Gb[2][c]};
+    double gb[3] = {0.0, 0.0, 0.0};
+    dirColumnVJP(F, h, qop, g, aF, aG, gb, B1b, Bmb, Blb, hb, qb);
+#pragma unroll
+    for (int q = 0; q < 3; ++q) GinB[q][c] = gb[q];
+  }
+  // ---- dFdL = h^2/6 (L1 + L2 + L3), dGdL = h/6 (L1 + 2 L2 + 2 L3 + L4)
+  double dFdLb[3], dGdLb[3];
 #pragma unroll
   for (int r = 0; r < 3; ++r) {
-#pragma unroll
-    for (int q = 0; q < 3; ++q) {
-      double sF = 0.0, sG = 0.0;
-#pragma unroll
-      for (int c = 0; c < 3; ++c) {
-        sF += Ab[r][2 + c] * Gin[q][c];
-        sG += Gb[r][c] * Gin[q][c];
-      }
-      dFdTb[r][q] = sF;
-      dGdTb[r][q] = sG;
-    }
     dFdLb[r] = Ab[r][4];
     dGdLb[r] = Gb[r][2];
   }
-#pragma unroll
-  for (int q = 0; q < 3; ++q)
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      double s = 0.0;
-#pragma unroll
-      for (int r = 0; r < 3; ++r) s += F.dFdT[r][q] * Ab[r][2 + c] + F.dGdT[r][q] * Gb[r][c];
-      GinB[q][c] = s;
-    }
-  // ---- dFdT = h I + h^2/6 (T1+T2+T3), dGdT = I + h/6 (T1 + 2T2 + 2T3 + T4), dFdL, dGdL
-  double T1b[3][3], T2b[3][3], T3b[3][3], T4b[3][3];
-#pragma unroll
-  for (int r = 0; r < 3; ++r)
-#pragma unroll
-    for (int q = 0; q < 3; ++q) {
-      const double sT = F.T1[r][q] + F.T2[r][q] + F.T3[r][q];
-      const double sG = F.T1[r][q] + 2.0 * (F.T2[r][q] + F.T3[r][q]) + F.T4[r][q];
-      hb += dFdTb[r][q] * (((r == q) ? 1.0 : 0.0) + (h / 3.0) * sT) + dGdTb[r][q] * (sG / 6.0);
-      const double a = (h * h / 6.0) * dFdTb[r][q], g = (h / 6.0) * dGdTb[r][q];
-      T1b[r][q] = a + g;
-      T2b[r][q] = a + 2.0 * g;
-      T3b[r][q] = a + 2.0 * g;
-      T4b[r][q] = g;
-    }
   double L1b[3], L2b[3], L3b[3], L4b[3];
 #pragma unroll
   for (int r = 0; r < 3; ++r) {
@@ -314,25 +286,6 @@ KFD_INLINE void stepVJP(const DevField &fld, const StepFwd &F, const double pos[
     L2b[r] = a + 2.0 * g;
     L3b[r] = a + 2.0 * g;
     L4b[r] = g;
-  }
-  // ---- T chain: T4 = qop M(Bl) X4, X4 = I + h T3; T3, T2 with M(Bm) and hh; T1 = qop M(B1)
-  {
-    double Ml[3][3], Mm[3][3], M1[3][3];
-    crossMat(F.Bl, Ml);
-    crossMat(F.Bm, Mm);
-    crossMat(F.B1, M1);
-    chainVJP(Ml, F.X4, F.T4, qop, T4b, Blb, qb, T3b, h, F.T3, hb, 1.0);
-    chainVJP(Mm, F.X3, F.T3, qop, T3b, Bmb, qb, T2b, hh, F.T2, hb, 0.5);
-    chainVJP(Mm, F.X2, F.T2, qop, T2b, Bmb, qb, T1b, hh, F.T1, hb, 0.5);
-    double Mb[3][3];
-#pragma unroll
-    for (int r = 0; r < 3; ++r)
-#pragma unroll
-      for (int q = 0; q < 3; ++q) {
-        qb += T1b[r][q] * M1[r][q];
-        Mb[r][q] = qop * T1b[r][q];
-      }
-    crossMatVJP(Mb, B1b);
   }
   // ---- L chain: L4 = t4 x Bl + (qop h)(L3 x Bl); L3, L2 with Bm and hh; L1 = d x B1
   {
