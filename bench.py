@@ -122,10 +122,15 @@ def cpu_baseline(batch, opts_c, threads, sample, gpu_results, others=None):
         threads = os.cpu_count() or 1
     oracle.time_fit(sub.subset(np.arange(min(n, 2000))), opts_c, threads)  # warm
     sec = oracle.time_fit(sub, opts_c, threads)
+    n1 = min(n, 50000)
+    sec1 = oracle.time_fit(sub.subset(np.arange(n1)), opts_c, 1)
     base = {"value": n / sec, "unit": "tracks/s", "cores": threads, "kind": "port",
             "sample": f"first {n} tracks of the same workload, FP64 oracle "
                       f"(oracle/kfo_fitter.hpp), OpenMP over tracks, {threads} threads, "
-                      f"fit loop timed alone ({sec:.2f} s)"}
+                      f"fit loop timed alone ({sec:.2f} s)",
+            "single_thread": {"value": n1 / sec1, "unit": "tracks/s",
+                              "sample": f"first {n1} tracks, 1 thread ({sec1:.2f} s)"},
+            "host": host_cpu()}
     ores, _, _, _ = oracle.fit(sub, opts_c, n_threads=threads)
     other_errors = {}
     for name, (ob, ores_gpu) in (others or {}).items():
@@ -133,6 +138,24 @@ def cpu_baseline(batch, opts_c, threads, sample, gpu_results, others=None):
         other_errors[name] = dict(param_errors(ores_gpu, o, ob.n_tracks),
                                   sample=f"first {ob.n_tracks} tracks")
     return base, param_errors(gpu_results, ores, n), other_errors
+
+
+def host_cpu():
+    """CPU model and the number of host cores the process may use."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count()
+    return {"model": model, "cores_usable": usable, "cores_online": os.cpu_count()}
 
 
 def param_errors(gpu_results, ores, n):
@@ -279,10 +302,25 @@ def material_leg(steps, warmup, sample):
     from gpukalmanfitter_amd import abi, configs
     from gpukalmanfitter_amd import generate as G
     from gpukalmanfitter_amd.fitter import DeviceBatch, KalmanFitter, KalmanFitterOptions
-    t0 = time.time()
+    t0 = time.perf_counter()
     b = G.generate(5)
-    print(f"[bench] generated {b.n_tracks} material tracks in {time.time() - t0:.1f}s",
+    cpu_gen_s = time.perf_counter() - t0
+    print(f"[bench] generated {b.n_tracks} material tracks in {cpu_gen_s:.1f}s",
           file=sys.stderr, flush=True)
+    # the same workload from the GPU generator (arrays left in HBM); the first
+    # call loads the library and warms up
+    G.generate_device(5, n_tracks=65536)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    w = G.generate_device(5)
+    torch.cuda.synchronize()
+    gpu_gen_s = time.perf_counter() - t0
+    same_hits = bool(w.n_hits == b.n_hits and np.array_equal(
+        w.tensors["hit_offset"].cpu().numpy(), b.hit_offset))
+    del w
+    torch.cuda.empty_cache()
+    print(f"[bench] GPU generator: {b.n_tracks} tracks in {gpu_gen_s * 1e3:.1f} ms "
+          f"(host {cpu_gen_s:.2f} s), hit offsets equal {same_hits}", file=sys.stderr, flush=True)
     kf = KalmanFitter(b.planes, b.field)
     dev = DeviceBatch(b)
 
@@ -298,7 +336,13 @@ def material_leg(steps, warmup, sample):
 
     out = {"workload": "config5-material: 4194304 tracks, U{8..40} planes of 300 um silicon at "
                        "25 mm pitch, p 0.5-20 GeV, Highland + Bethe, 101x101x201 FP32 grid field",
-           "memory_budget_bytes": 16 << 30}
+           "memory_budget_bytes": 16 << 30,
+           "generator": {"gpu_s": gpu_gen_s, "gpu_tracks_per_s": b.n_tracks / gpu_gen_s,
+                         "host_s": cpu_gen_s, "host_tracks_per_s": b.n_tracks / cpu_gen_s,
+                         "host_threads": os.cpu_count(), "hit_offsets_equal": same_hits,
+                         "note": "seeded simulation + hit packing; host = libkfgen.so (OpenMP) "
+                                 "incl. Python array setup, GPU = libkfgen_gpu.so incl. one "
+                                 "device sync for the hit count"}}
     res = {}
     rates = {}
     for prec in ("mixed", "fp64"):

@@ -1,15 +1,9 @@
 // kf_generate.cpp -- seeded synthetic track generator (host, libkfgen.so).
 //
-// Produces the fitter's inputs for a telescope of x-normal planes: particle
-// generation, truth propagation with an adaptive RK4 (Nystroem) integrator,
-// optional material effects (sampled Highland scattering and Bethe energy
-// loss with Gaussian straggling), hit smearing and start-parameter smearing.
-// It restates the shape of the reference harness (Run/Common/Processor.hpp:
-// 40-253, include/ActsExamples/ParametricParticleGenerator.hpp:86-140) with a
-// counter-based RNG so that a sample does not depend on thread count or
-// platform: every random number is Philox4x32-10 (Salmon et al., SC'11,
-// "Parallel random numbers: as easy as 1, 2, 3") of (seed, track, stream,
-// draw).  Normal deviates use the Box-Muller transform.
+// The per-track simulation (particle generation, truth propagation, material
+// effects, smearing) is kf_gen_track.hpp, shared with the device generator
+// (kf_generate_gpu.hip); this file runs it over tracks with OpenMP and holds
+// the plane table, misalignment, hit packing and grid-field helpers.
 #include <omp.h>
 
 #include <algorithm>
@@ -18,200 +12,11 @@
 #include <cstring>
 #include <vector>
 
-#include "kf_generate.h"
+#define KFG_HD
+#include "kf_gen_track.hpp"
 
-namespace {
-
-// ------------------------------------------------------------ Philox4x32 ----
-struct Philox {
-  uint32_t k0, k1;
-  static inline void mulhilo(uint32_t a, uint32_t b, uint32_t &hi, uint32_t &lo) {
-    const uint64_t p = (uint64_t)a * b;
-    hi = (uint32_t)(p >> 32);
-    lo = (uint32_t)p;
-  }
-  // 10 rounds, multipliers 0xD2511F53 / 0xCD9E8D57, Weyl keys 0x9E3779B9 / 0xBB67AE85
-  void gen(uint32_t c[4]) const {
-    uint32_t a0 = k0, a1 = k1;
-    for (int r = 0; r < 10; ++r) {
-      uint32_t h0, l0, h1, l1;
-      mulhilo(0xD2511F53u, c[0], h0, l0);
-      mulhilo(0xCD9E8D57u, c[2], h1, l1);
-      const uint32_t n0 = h1 ^ c[1] ^ a0;
-      const uint32_t n1 = l1;
-      const uint32_t n2 = h0 ^ c[3] ^ a1;
-      const uint32_t n3 = l0;
-      c[0] = n0;
-      c[1] = n1;
-      c[2] = n2;
-      c[3] = n3;
-      a0 += 0x9E3779B9u;
-      a1 += 0xBB67AE85u;
-    }
-  }
-};
-
-// Per-track random stream: uniform doubles in (0, 1) and N(0, 1).
-struct TrackRng {
-  Philox ph;
-  uint32_t track, stream;
-  uint32_t draw = 0;
-  uint32_t buf[4];
-  int left = 0;
-  uint32_t next32() {
-    if (left == 0) {
-      buf[0] = draw++;
-      buf[1] = stream;
-      buf[2] = track;
-      buf[3] = 0x6B66u;  // domain tag
-      ph.gen(buf);
-      left = 4;
-    }
-    return buf[4 - (left--)];
-  }
-  double uniform() {  // 53-bit, strictly inside (0, 1)
-    const uint64_t a = next32() >> 5, b = next32() >> 6;
-    return ((double)(a * 67108864ull + b) + 0.5) / 9007199254740992.0;
-  }
-  double normal() {
-    const double u1 = uniform(), u2 = uniform();
-    return std::sqrt(-2.0 * std::log(u1)) * std::cos(6.283185307179586 * u2);
-  }
-};
-
-// ------------------------------------------------------------------ field ----
-struct GenField {
-  int grid = 0;
-  double b[3] = {0, 0, 0};
-  const float *v = nullptr;
-  int n[3] = {0, 0, 0};
-  double lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
-  double scale = 1.0;
-  void get(const double p[3], double out[3]) const {
-    if (!grid) {
-      for (int k = 0; k < 3; ++k) out[k] = scale * b[k];
-      return;
-    }
-    int idx[3];
-    double f[3];
-    for (int d = 0; d < 3; ++d) {
-      double x = std::min(std::max(p[d], lo[d]), hi[d]);
-      const double w = (hi[d] - lo[d]) / (n[d] - 1);
-      int i = (int)std::floor((x - lo[d]) / w);
-      i = std::max(0, std::min(i, n[d] - 2));
-      idx[d] = i;
-      f[d] = (x - (lo[d] + i * w)) / w;
-    }
-    double acc[3] = {0, 0, 0};
-    for (int c = 0; c < 8; ++c) {
-      const int dx = (c >> 2) & 1, dy = (c >> 1) & 1, dz = c & 1;
-      const double wgt = (dx ? f[0] : 1 - f[0]) * (dy ? f[1] : 1 - f[1]) * (dz ? f[2] : 1 - f[2]);
-      const size_t o = (((size_t)(idx[0] + dx) * n[1] + idx[1] + dy) * n[2] + idx[2] + dz) * 3;
-      for (int k = 0; k < 3; ++k) acc[k] += wgt * v[o + k];
-    }
-    for (int k = 0; k < 3; ++k) out[k] = scale * acc[k];
-  }
-};
-
-inline void cross3(const double a[3], const double b[3], double r[3]) {
-  r[0] = a[1] * b[2] - a[2] * b[1];
-  r[1] = a[2] * b[0] - a[0] * b[2];
-  r[2] = a[0] * b[1] - a[1] * b[0];
-}
-inline double dot3(const double a[3], const double b[3]) {
-  return a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
-}
-
-// One RK4-Nystroem step of length h for d(dir)/ds = qop * dir x B.
-// Returns the embedded error estimate h^2 |k1 - k2 - k3 + k4|_1.
-double rkStep(const GenField &fld, double qop, double pos[3], double dir[3], double h) {
-  double B[3], k1[3], k2[3], k3[3], k4[3], t[3], p1[3];
-  fld.get(pos, B);
-  cross3(dir, B, k1);
-  for (int i = 0; i < 3; ++i) k1[i] *= qop;
-  for (int i = 0; i < 3; ++i) p1[i] = pos[i] + 0.5 * h * dir[i] + 0.125 * h * h * k1[i];
-  fld.get(p1, B);
-  for (int i = 0; i < 3; ++i) t[i] = dir[i] + 0.5 * h * k1[i];
-  cross3(t, B, k2);
-  for (int i = 0; i < 3; ++i) k2[i] *= qop;
-  for (int i = 0; i < 3; ++i) t[i] = dir[i] + 0.5 * h * k2[i];
-  cross3(t, B, k3);
-  for (int i = 0; i < 3; ++i) k3[i] *= qop;
-  for (int i = 0; i < 3; ++i) p1[i] = pos[i] + h * dir[i] + 0.5 * h * h * k3[i];
-  fld.get(p1, B);
-  for (int i = 0; i < 3; ++i) t[i] = dir[i] + h * k3[i];
-  cross3(t, B, k4);
-  for (int i = 0; i < 3; ++i) k4[i] *= qop;
-  double err = 0;
-  for (int i = 0; i < 3; ++i) err += std::fabs(k1[i] - k2[i] - k3[i] + k4[i]);
-  for (int i = 0; i < 3; ++i) {
-    pos[i] += h * dir[i] + h * h / 6.0 * (k1[i] + k2[i] + k3[i]);
-    dir[i] += h / 6.0 * (k1[i] + 2.0 * (k2[i] + k3[i]) + k4[i]);
-  }
-  const double dn = std::sqrt(dot3(dir, dir));
-  for (int i = 0; i < 3; ++i) dir[i] /= dn;
-  return h * h * err;
-}
-
-// Propagates to the plane (centre c, normal n).  Returns false if the plane is
-// not reached (the track turns away first).  Path length is added to *path.
-bool propagateToPlane(const GenField &fld, double qop, double pos[3], double dir[3],
-                      const double c[3], const double n[3], double *path) {
-  const double hmax = 20.0;
-  for (int it = 0; it < 100000; ++it) {
-    const double denom = dot3(dir, n);
-    double d[3] = {c[0] - pos[0], c[1] - pos[1], c[2] - pos[2]};
-    const double dist = dot3(n, d);  // signed distance along the normal
-    if (std::fabs(dist) < 1e-7) return true;
-    if (denom <= 1e-3 && dist > 0) return false;  // moving away or grazing
-    double s = dist / denom;
-    if (s < -1.0) return false;
-    double h = std::max(-hmax, std::min(s, hmax));
-    // step-doubling error control
-    for (int tries = 0; tries < 60; ++tries) {
-      double p2[3] = {pos[0], pos[1], pos[2]}, d2[3] = {dir[0], dir[1], dir[2]};
-      const double err = rkStep(fld, qop, p2, d2, h);
-      if (err <= 1e-7 || std::fabs(h) < 1e-3) {
-        std::memcpy(pos, p2, sizeof p2);
-        std::memcpy(dir, d2, sizeof d2);
-        *path += h;
-        break;
-      }
-      h *= 0.5;
-    }
-  }
-  return false;
-}
-
-// RPP formulas for the material simulation (same physics the fit models:
-// Bethe mean loss, Highland width, Landau-width Gaussian straggling).
-struct GenMaterial {
-  double thickness, x0, ar, z, molarRho;
-};
-constexpr double kMe = 0.5109989461e-3;
-constexpr double kK = 0.307075e-3 * 100.0;
-void materialEffects(const GenMaterial &m, double mass, double p, double pathCorr,
-                     double *meanLoss, double *sigmaE, double *theta0) {
-  const double x = m.thickness * pathCorr;
-  const double q2OverBeta2 = 1.0 + (mass / p) * (mass / p);
-  const double beta2 = 1.0 / q2OverBeta2;
-  const double bg = p / mass;
-  const double gamma = std::sqrt(1.0 + bg * bg);
-  const double Ne = m.z * m.molarRho;
-  const double I = 16e-9 * std::pow(m.z, 0.9);
-  const double eps = 0.5 * kK * Ne * x * q2OverBeta2;
-  const double u = 2 * kMe * bg * bg;
-  const double mfrac = kMe / mass;
-  const double wmax = 2 * kMe * bg * bg / (1.0 + 2 * gamma * mfrac + mfrac * mfrac);
-  double dhalf = 0;
-  if (bg >= 10.0) dhalf = std::log(bg) + std::log(28.816e-9 * std::sqrt(Ne) / I) - 0.5;
-  *meanLoss = eps * (0.5 * std::log(u / I) + 0.5 * std::log(wmax / I) - beta2 - dhalf);
-  *sigmaE = 4 * eps / (2 * std::sqrt(2 * std::log(2.0)));
-  const double t = std::sqrt(x / m.x0 * q2OverBeta2);
-  *theta0 = 13.6e-3 / p * t * (1.0 + 0.038 * 2 * std::log(t));
-}
-
-}  // namespace
+using kfg::Philox;
+using kfg::TrackRng;
 
 extern "C" {
 
@@ -246,7 +51,7 @@ void kfg_planes(const kfg_config *cfg, kf_plane *planes) {
 void kfg_misalign(const kfg_config *cfg, kf_plane *planes, uint64_t seed) {
   Philox ph{(uint32_t)seed, (uint32_t)(seed >> 32)};
   for (int i = 0; i < cfg->n_planes; ++i) {
-    TrackRng r{ph, (uint32_t)i, 77u};
+    TrackRng r(ph, (uint32_t)i, 77u);
     double sh[3], an[3];
     for (int k = 0; k < 3; ++k) sh[k] = cfg->align_shift_sigma * r.normal();
     for (int k = 0; k < 3; ++k) an[k] = cfg->align_rot_sigma * r.normal();
@@ -293,8 +98,7 @@ int kfg_simulate(const kfg_config *cfg, const kf_plane *true_planes,
                  int32_t *n_hits, double *scratch_u, double *scratch_v,
                  int32_t *scratch_geo, double *truth, int32_t n_threads) {
   const int64_t n = cfg->n_tracks;
-  const int maxp = cfg->max_planes;
-  GenField fld;
+  kfg::GenField fld;
   fld.grid = field->grid;
   for (int k = 0; k < 3; ++k) {
     fld.b[k] = field->b[k];
@@ -305,105 +109,11 @@ int kfg_simulate(const kfg_config *cfg, const kf_plane *true_planes,
   fld.v = field->values;
   fld.scale = cfg->true_field_scale;
   const Philox ph{(uint32_t)cfg->seed, (uint32_t)(cfg->seed >> 32)};
+  const kfg::SimOut o{start_params, start_cov, perigee, seq_first, seq_count,
+                      n_hits,       scratch_u, scratch_v, scratch_geo, truth};
   if (n_threads <= 0) n_threads = omp_get_max_threads();
-
 #pragma omp parallel for num_threads(n_threads) schedule(dynamic, 256)
-  for (int64_t it = 0; it < n; ++it) {
-    TrackRng r{ph, (uint32_t)it, 1u};
-    const double q = (r.uniform() < 0.5) ? -1.0 : 1.0;
-    const double p = cfg->p_min + (cfg->p_max - cfg->p_min) * r.uniform();
-    const double phi = cfg->phi_min + (cfg->phi_max - cfg->phi_min) * r.uniform();
-    const double eta = cfg->eta_min + (cfg->eta_max - cfg->eta_min) * r.uniform();
-    const double theta = 2.0 * std::atan(std::exp(-eta));
-    double vtx[3];
-    for (int k = 0; k < 3; ++k) vtx[k] = cfg->vertex_sigma * r.normal();
-    int cnt = cfg->min_planes;
-    if (cfg->max_planes > cfg->min_planes)
-      cnt += (int)(r.uniform() * (cfg->max_planes - cfg->min_planes + 1));
-    cnt = std::min(cnt, std::min(cfg->max_planes, cfg->n_planes));
-    seq_first[it] = 0;
-    seq_count[it] = cnt;
-    for (int k = 0; k < 3; ++k) perigee[k * n + it] = vtx[k];
-    if (truth) {
-      const double tv[6] = {0.0, 0.0, phi, theta, q / p, 0.0};
-      for (int k = 0; k < 6; ++k) truth[k * n + it] = tv[k];
-    }
-
-    // truth propagation through the (true) planes
-    TrackRng rs{ph, (uint32_t)it, 2u};
-    double pos[3] = {vtx[0], vtx[1], vtx[2]};
-    double dir[3] = {std::cos(phi) * std::sin(theta), std::sin(phi) * std::sin(theta),
-                     std::cos(theta)};
-    double pp = p, path = 0;
-    int nh = 0;
-    for (int k = 0; k < cnt; ++k) {
-      const kf_plane &pl = true_planes[k];
-      const double nrm[3] = {pl.rot[2], pl.rot[5], pl.rot[8]};
-      if (!propagateToPlane(fld, q / pp, pos, dir, pl.center, nrm, &path)) break;
-      // local coordinates of the crossing
-      double d[3] = {pos[0] - pl.center[0], pos[1] - pl.center[1], pos[2] - pl.center[2]};
-      const double lu = pl.rot[0] * d[0] + pl.rot[3] * d[1] + pl.rot[6] * d[2];
-      const double lv = pl.rot[1] * d[0] + pl.rot[4] * d[1] + pl.rot[7] * d[2];
-      const double su = cfg->hit_sigma_u * rs.normal();
-      const double sv = cfg->hit_sigma_v * rs.normal();
-      const bool drop = (cfg->hole_prob > 0.0) && (rs.uniform() < cfg->hole_prob);
-      if (!drop) {
-        scratch_u[it * maxp + nh] = lu + su;
-        scratch_v[it * maxp + nh] = lv + sv;
-        scratch_geo[it * maxp + nh] = pl.geo_id;
-        ++nh;
-      }
-      if (cfg->material && pl.thickness > 0.0) {
-        GenMaterial m{pl.thickness, pl.x0, pl.ar, pl.z, pl.molar_rho};
-        const double pc = 1.0 / std::fabs(dot3(nrm, dir));
-        double meanLoss, sigmaE, theta0;
-        materialEffects(m, cfg->mass, pp, pc, &meanLoss, &sigmaE, &theta0);
-        double loss = meanLoss + sigmaE * rs.normal();
-        if (loss < 0) loss = 0;
-        const double E = std::sqrt(cfg->mass * cfg->mass + pp * pp) - loss;
-        if (E <= cfg->mass * 1.0001) break;  // stopped
-        pp = std::sqrt(E * E - cfg->mass * cfg->mass);
-        // scattering: two orthogonal Gaussian deflections
-        double a[3] = {0, 0, 1};
-        if (std::fabs(dir[2]) > 0.9) a[0] = 1, a[2] = 0;
-        double e1[3], e2[3];
-        cross3(dir, a, e1);
-        double l1 = std::sqrt(dot3(e1, e1));
-        for (int i = 0; i < 3; ++i) e1[i] /= l1;
-        cross3(dir, e1, e2);
-        const double t1 = theta0 * rs.normal(), t2 = theta0 * rs.normal();
-        for (int i = 0; i < 3; ++i) dir[i] += t1 * e1[i] + t2 * e2[i];
-        const double dn = std::sqrt(dot3(dir, dir));
-        for (int i = 0; i < 3; ++i) dir[i] /= dn;
-      }
-    }
-    n_hits[it] = nh;
-
-    // start parameters: truth at the perigee, smeared (Processor.hpp:177-253)
-    TrackRng rp{ph, (uint32_t)it, 3u};
-    const double sp_ = cfg->sigma_p_rel * p;
-    double sp[6];
-    sp[0] = cfg->sigma_d0 * rp.normal();
-    sp[1] = cfg->sigma_z0 * rp.normal();
-    sp[5] = cfg->sigma_t * rp.normal();
-    double nphi = phi + cfg->sigma_phi * rp.normal();
-    double nth = theta + cfg->sigma_theta * rp.normal();
-    if (nth < 0) nth = -nth, nphi += M_PI;       // normalizePhiTheta
-    if (nth > M_PI) nth = 2 * M_PI - nth, nphi += M_PI;
-    nphi = nphi - 2 * M_PI * std::floor((nphi + M_PI) / (2 * M_PI));
-    sp[2] = nphi;
-    sp[3] = nth;
-    const double np = std::max(0.0, p + sp_ * rp.normal());
-    sp[4] = q / np;
-    for (int k = 0; k < 6; ++k) start_params[k * n + it] = sp[k];
-    const double sqop = sp_ / (p * p);
-    const double diag[6] = {cfg->sigma_d0 * cfg->sigma_d0, cfg->sigma_z0 * cfg->sigma_z0,
-                            cfg->sigma_phi * cfg->sigma_phi, cfg->sigma_theta * cfg->sigma_theta,
-                            sqop * sqop, cfg->sigma_t * cfg->sigma_t};
-    int kk = 0;
-    for (int a = 0; a < 6; ++a)
-      for (int b = a; b < 6; ++b) start_cov[(int64_t)(kk++) * n + it] = (a == b) ? diag[a] : 0.0;
-  }
+  for (int64_t it = 0; it < n; ++it) kfg::simulateTrack(*cfg, true_planes, fld, ph, it, o);
   return 0;
 }
 
@@ -414,21 +124,12 @@ int kfg_pack_hits(const kfg_config *cfg, const int32_t *n_hits, const int64_t *h
                   const int32_t *scratch_geo, double sigma_u, double sigma_v,
                   int32_t *hit_geo, double *hit_meas, double *hit_cov, int32_t n_threads) {
   const int64_t n = cfg->n_tracks;
-  const int maxp = cfg->max_planes;
   const int64_t m = hit_offset[n];
   if (n_threads <= 0) n_threads = omp_get_max_threads();
 #pragma omp parallel for num_threads(n_threads) schedule(static)
-  for (int64_t it = 0; it < n; ++it) {
-    for (int k = 0; k < n_hits[it]; ++k) {
-      const int64_t h = hit_offset[it] + k;
-      hit_geo[h] = scratch_geo[it * maxp + k];
-      hit_meas[h] = scratch_u[it * maxp + k];
-      hit_meas[m + h] = scratch_v[it * maxp + k];
-      hit_cov[h] = sigma_u * sigma_u;
-      hit_cov[m + h] = 0.0;
-      hit_cov[2 * m + h] = sigma_v * sigma_v;
-    }
-  }
+  for (int64_t it = 0; it < n; ++it)
+    kfg::packTrack(*cfg, n_hits, hit_offset, m, scratch_u, scratch_v, scratch_geo, sigma_u,
+                   sigma_v, hit_geo, hit_meas, hit_cov, it);
   return 0;
 }
 

@@ -63,6 +63,27 @@ void kfg_fill_grid(float *values, int32_t nx, int32_t ny, int32_t nz, const doub
 // Philox4x32-10 of (counter[4], key[2]) in place.
 void kfg_philox(uint32_t *counter, const uint32_t *key);
 
+// The same generator on the GPU (libkfgen_gpu.so, kf_generate_gpu.hip): one
+// thread per track, asynchronous on `stream` (a hipStream_t); every pointer is
+// device memory, including true_planes [n_planes] and field->values.  Return
+// 0 or a negative code.
+int kfg_simulate_device(const kfg_config *cfg, const kf_plane *true_planes, const kfg_field *field,
+                        double *start_params, double *start_cov, double *perigee,
+                        int32_t *seq_first, int32_t *seq_count, int32_t *n_hits,
+                        double *scratch_u, double *scratch_v, int32_t *scratch_geo,
+                        double *truth, void *stream);
+// hit_offset[0..n] = exclusive prefix sum of n_hits[0..n-1] (hit_offset[n] =
+// the number of hits); scratch: kfg_offsets_scratch_bytes(n) bytes of device
+// memory.
+int64_t kfg_offsets_scratch_bytes(int64_t n);
+int kfg_offsets_device(int64_t n, const int32_t *n_hits, int64_t *hit_offset, void *scratch,
+                       void *stream);
+// kfg_pack_hits on the GPU; n_hits_total = hit_offset[n].
+int kfg_pack_hits_device(const kfg_config *cfg, const int32_t *n_hits, const int64_t *hit_offset,
+                         int64_t n_hits_total, const double *scratch_u, const double *scratch_v,
+                         const int32_t *scratch_geo, double sigma_u, double sigma_v,
+                         int32_t *hit_geo, double *hit_meas, double *hit_cov, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

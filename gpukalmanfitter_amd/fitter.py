@@ -191,6 +191,22 @@ class DeviceBatch:
             if arr.size == 0:
                 arr = np.zeros(1, dtype=arr.dtype)
             self.t[name] = torch.from_numpy(arr).to(self.device)
+        self._outputs(track_states, max_states if max_states else max(batch.max_states, 1))
+
+    @classmethod
+    def from_tensors(cls, tensors, n_tracks, n_hits, max_states, track_states=False):
+        """A batch whose inputs are already device tensors (the GPU generator's
+        output), in the layout of kf_tracks; no copy."""
+        self = cls.__new__(cls)
+        self.t = {name: tensors[name] for name in cls._INPUTS}
+        self.device = self.t["start_params"].device
+        self.n_tracks = int(n_tracks)
+        self.n_hits = int(n_hits)
+        self._outputs(track_states, max(int(max_states), 1))
+        return self
+
+    def _outputs(self, track_states, max_states):
+        torch = _torch()
         n = max(self.n_tracks, 1)
         f64 = dict(dtype=torch.float64, device=self.device)
         self.out = {
@@ -201,7 +217,7 @@ class DeviceBatch:
             "n_measured": torch.zeros(n, dtype=torch.int32, device=self.device),
             "status": torch.zeros(n, dtype=torch.int32, device=self.device),
         }
-        self.max_states = int(max_states if max_states else max(batch.max_states, 1))
+        self.max_states = int(max_states)
         if track_states:
             self.out["track_states"] = torch.zeros(
                 (self.max_states, abi.KF_TS_STRIDE, n), **f64)

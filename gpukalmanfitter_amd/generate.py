@@ -1,4 +1,4 @@
-"""Seeded synthetic workloads (host side, libkfgen.so).
+"""Seeded synthetic workloads (libkfgen.so on the host, libkfgen_gpu.so on the GPU).
 
 ``generate(config_id, n_tracks=None)`` returns a :class:`Batch` holding the
 plane table, the field and every per-track / per-hit array in the layout of
@@ -140,8 +140,8 @@ def make_field(spec):
     return FieldSpec(kind="grid", values=vals, lo=tuple(lo), hi=tuple(hi))
 
 
-def generate(config_id, n_tracks=None, seed=None, n_threads=0, **overrides):
-    """Generates the workload of config ``config_id`` (1..5)."""
+def _setup(config_id, n_tracks, seed, overrides):
+    """Generator configuration, nominal and true plane tables, field."""
     spec = CONFIGS[config_id]
     gen = dict(spec["gen"])
     if n_tracks is not None:
@@ -156,7 +156,6 @@ def generate(config_id, n_tracks=None, seed=None, n_threads=0, **overrides):
         gen["true_field_scale"] = 1.0 + spec["field_scale_sigma"] * float(rs.standard_normal())
     cfg = kfg_config(**gen)
     lib = _lib()
-    n = cfg.n_tracks
     planes = (abi.kf_plane * cfg.n_planes)()
     lib.kfg_planes(C.byref(cfg), planes)
     true_planes = (abi.kf_plane * cfg.n_planes)()
@@ -175,7 +174,14 @@ def generate(config_id, n_tracks=None, seed=None, n_threads=0, **overrides):
     else:
         for k in range(3):
             fld.b[k] = field.b[k]
+    return cfg, planes, true_planes, field, fld
 
+
+def generate(config_id, n_tracks=None, seed=None, n_threads=0, **overrides):
+    """Generates the workload of config ``config_id`` (1..5)."""
+    cfg, planes, true_planes, field, fld = _setup(config_id, n_tracks, seed, overrides)
+    lib = _lib()
+    n = cfg.n_tracks
     maxp = cfg.max_planes
     start_params = np.zeros((6, n))
     start_cov = np.zeros((21, n))
@@ -215,3 +221,122 @@ def philox4x32_10(counter, key):
     k = np.array(key, dtype=np.uint32)
     _lib().kfg_philox(_ptr(c), _ptr(k))
     return [int(x) for x in c]
+
+
+def _gpu_lib():
+    lib = load_library("libkfgen_gpu.so")
+    if not getattr(lib, "_kfg_typed", False):
+        P = C.c_void_p
+        lib.kfg_simulate_device.argtypes = ([C.POINTER(kfg_config), P, C.POINTER(kfg_field)] +
+                                            [P] * 11)
+        lib.kfg_simulate_device.restype = C.c_int
+        lib.kfg_offsets_scratch_bytes.argtypes = [C.c_int64]
+        lib.kfg_offsets_scratch_bytes.restype = C.c_int64
+        lib.kfg_offsets_device.argtypes = [C.c_int64, P, P, P, P]
+        lib.kfg_offsets_device.restype = C.c_int
+        lib.kfg_pack_hits_device.argtypes = ([C.POINTER(kfg_config), P, P, C.c_int64, P, P, P,
+                                              C.c_double, C.c_double, P, P, P, P])
+        lib.kfg_pack_hits_device.restype = C.c_int
+        lib._kfg_typed = True
+    return lib
+
+
+@dataclasses.dataclass
+class DeviceWorkload:
+    """A workload generated on the GPU: the host-side geometry and field, and
+    the per-track / per-hit arrays as device tensors in the layout of
+    ``kf_tracks`` (``tensors``), plus the truth parameters."""
+    planes: object
+    true_planes: object
+    field: FieldSpec
+    tensors: dict
+    n_tracks: int
+    n_hits: int
+    max_states: int
+    config_id: int = 0
+    true_field_scale: float = 1.0
+
+    def device_batch(self, track_states=False):
+        from .fitter import DeviceBatch
+        return DeviceBatch.from_tensors(self.tensors, self.n_tracks, self.n_hits,
+                                        self.max_states, track_states=track_states)
+
+    def to_host(self):
+        """The same workload as a host :class:`Batch` (copies)."""
+        h = {k: v.cpu().numpy() for k, v in self.tensors.items()}
+        return Batch(planes=self.planes, true_planes=self.true_planes, field=self.field,
+                     start_params=h["start_params"], start_cov=h["start_cov"],
+                     perigee=h["perigee"], seq_first=h["seq_first"], seq_count=h["seq_count"],
+                     hit_offset=h["hit_offset"], hit_geo_id=h["hit_geo_id"],
+                     hit_meas=h["hit_meas"], hit_cov=h["hit_cov"], truth=h["truth"],
+                     config_id=self.config_id, true_field_scale=self.true_field_scale)
+
+
+def generate_device(config_id, n_tracks=None, seed=None, device="cuda:0", stream=None,
+                    **overrides):
+    """Generates the workload of config ``config_id`` on the GPU
+    (libkfgen_gpu.so): the same tracks as :func:`generate` (identical random
+    streams; values agree to the rounding of the device math library), with
+    every array left in device memory.  Synchronises once to size the hit
+    arrays."""
+    import torch
+    if not torch.cuda.is_available():
+        raise RuntimeError("generate_device needs a HIP device")
+    cfg, planes, true_planes, field, fld = _setup(config_id, n_tracks, seed, overrides)
+    lib = _gpu_lib()
+    dev = torch.device(device)
+    s = stream if stream is not None else torch.cuda.current_stream(dev)
+    sp = C.c_void_p(s.cuda_stream)
+    n = cfg.n_tracks
+    maxp = cfg.max_planes
+    f64 = dict(dtype=torch.float64, device=dev)
+    i32 = dict(dtype=torch.int32, device=dev)
+    nn = max(n, 1)
+    t = {"start_params": torch.empty((6, nn), **f64), "start_cov": torch.empty((21, nn), **f64),
+         "perigee": torch.empty((3, nn), **f64), "seq_first": torch.empty(nn, **i32),
+         "seq_count": torch.empty(nn, **i32), "truth": torch.empty((6, nn), **f64)}
+    n_hits = torch.zeros(nn, **i32)
+    su = torch.empty((nn, maxp), **f64)
+    sv = torch.empty((nn, maxp), **f64)
+    sg = torch.empty((nn, maxp), **i32)
+    planes_d = torch.frombuffer(bytearray(bytes(true_planes)), dtype=torch.uint8).to(dev)
+    grid_d = None
+    if field.kind == "grid":
+        grid_d = torch.from_numpy(np.ascontiguousarray(field.values)).to(dev)
+        fld.values = grid_d.data_ptr()
+    with torch.cuda.stream(s):
+        rc = lib.kfg_simulate_device(
+            C.byref(cfg), planes_d.data_ptr(), C.byref(fld),
+            *[t[k].data_ptr() for k in ("start_params", "start_cov", "perigee", "seq_first",
+                                         "seq_count")],
+            n_hits.data_ptr(), su.data_ptr(), sv.data_ptr(), sg.data_ptr(),
+            t["truth"].data_ptr(), sp)
+        if rc != 0:
+            raise RuntimeError(f"kfg_simulate_device failed ({rc})")
+        hit_offset = torch.empty(n + 1, dtype=torch.int64, device=dev)
+        scratch = torch.empty(max(int(lib.kfg_offsets_scratch_bytes(n)), 8), dtype=torch.uint8,
+                              device=dev)
+        rc = lib.kfg_offsets_device(n, n_hits.data_ptr(), hit_offset.data_ptr(),
+                                    scratch.data_ptr(), sp)
+        if rc != 0:
+            raise RuntimeError(f"kfg_offsets_device failed ({rc})")
+        m = int(hit_offset[n].item())
+        mm = max(m, 1)
+        hit_geo = torch.empty(mm, **i32)
+        hit_meas = torch.empty((2, mm), **f64)
+        hit_cov = torch.empty((3, mm), **f64)
+        rc = lib.kfg_pack_hits_device(C.byref(cfg), n_hits.data_ptr(), hit_offset.data_ptr(), m,
+                                      su.data_ptr(), sv.data_ptr(), sg.data_ptr(),
+                                      cfg.hit_sigma_u, cfg.hit_sigma_v, hit_geo.data_ptr(),
+                                      hit_meas.data_ptr(), hit_cov.data_ptr(), sp)
+        if rc != 0:
+            raise RuntimeError(f"kfg_pack_hits_device failed ({rc})")
+    s.synchronize()
+    del su, sv, sg, scratch, planes_d, grid_d
+    t.update(hit_offset=hit_offset, hit_geo_id=hit_geo[:m] if m else hit_geo,
+             hit_meas=hit_meas[:, :m].contiguous() if m else hit_meas,
+             hit_cov=hit_cov[:, :m].contiguous() if m else hit_cov)
+    max_states = int(t["seq_count"].max().item()) if n else 0
+    return DeviceWorkload(planes=planes, true_planes=true_planes, field=field, tensors=t,
+                          n_tracks=n, n_hits=m, max_states=max_states, config_id=config_id,
+                          true_field_scale=cfg.true_field_scale)
