@@ -85,6 +85,8 @@ def parse():
                     help="skip the ragged-batch leg (config 3, 2,000,000 tracks)")
     ap.add_argument("--no-material", action="store_true",
                     help="skip the 4,194,304-track material workload leg")
+    ap.add_argument("--no-intersect", action="store_true",
+                    help="skip the batched ray x triangle intersection leg")
     ap.add_argument("--no-adjoint", action="store_true",
                     help="skip the chi2-adjoint leg (config 4, 262,144 tracks)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -371,6 +373,72 @@ def material_leg(steps, warmup, sample):
     return out, b.subset(np.arange(min(sample, b.n_tracks))), res
 
 
+def intersect_leg(steps, warmup, n_rays=16384, cpu_rays=2048):
+    """Batched ray x triangle intersection (kf_intersect): rays from the
+    beam spot against a 10-layer barrel of about 37k triangles (the size of
+    the reference's TrackML mesh), boundary check on.  Device time from HIP
+    events around the call on its stream.  Returns the figures and the first
+    `cpu_rays` rays with their GPU hits, for the CPU baseline."""
+    import numpy as np
+    import torch
+    from gpukalmanfitter_amd import intersect as I
+    surfaces = I.polygon_planes(I.barrel_mesh())
+    p, d = I.make_rays(n_rays, seed=17)
+    ix = I.Intersector(surfaces)
+    pos = torch.from_numpy(p).cuda()
+    dirs = torch.from_numpy(d).cuda()
+    cap = 64 * n_rays
+    out = torch.empty(cap * I.HIT_DTYPE.itemsize, dtype=torch.uint8, device="cuda:0")
+    lib = I._typed()
+    scratch = torch.empty(int(lib.kf_intersect_scratch_bytes(n_rays)), dtype=torch.uint8,
+                          device="cuda:0")
+    n_hits = torch.zeros(1, dtype=torch.int64, device="cuda:0")
+
+    def call():
+        ix.run_device(pos, dirs, n_rays, True, cap, None, out, scratch, n_hits, False)
+
+    for _ in range(warmup):
+        call()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(steps):
+        call()
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / steps
+    total = int(n_hits.item())
+    pairs = n_rays * len(surfaces)
+    got = np.frombuffer(out[:min(total, cap) * I.HIT_DTYPE.itemsize].cpu().numpy().tobytes(),
+                        dtype=I.HIT_DTYPE)
+    res = {"workload": f"{n_rays} rays x {len(surfaces)} triangles (10-layer barrel, "
+                       "ConvexPolygonBounds<3>), boundary check",
+           "ms_per_call": ms, "pair_tests_per_s": pairs / (ms * 1e-3),
+           "rays_per_s": n_rays / (ms * 1e-3), "hits": total, "hits_per_ray": total / n_rays,
+           "dtype": "f64"}
+    print(f"[bench] intersect: {res['pair_tests_per_s']:.4g} pair tests/s ({ms:.3f} ms, "
+          f"{total} hits)", file=sys.stderr, flush=True)
+    return res, (surfaces, p[:, :cpu_rays], d[:, :cpu_rays], got[got["ray"] < cpu_rays])
+
+
+def intersect_cpu_baseline(surfaces, p, d, got, threads):
+    """The intersection oracle (oracle/kfo_intersect.cpp, OpenMP over rays)
+    timed on the ray sample, and whether its hit list (pairs, paths,
+    positions) is byte-identical to the GPU's on that sample."""
+    import numpy as np
+    import oracle
+    threads = threads if threads > 0 else (os.cpu_count() or 1)
+    p, d = np.ascontiguousarray(p), np.ascontiguousarray(d)
+    oracle.time_intersect(surfaces, p[:, :64], d[:, :64], True, threads)  # warm
+    sec = oracle.time_intersect(surfaces, p, d, True, threads)
+    ref = oracle.intersect(surfaces, p, d, True, threads)
+    same = len(got) == len(ref) and got.tobytes() == ref.tobytes()
+    n = p.shape[1]
+    return {"pair_tests_per_s": n * len(surfaces) / sec, "cores": threads, "kind": "port",
+            "sample": f"first {n} rays, oracle (oracle/kfo_intersect.cpp), OpenMP over rays "
+                      f"({sec:.2f} s)"}, same
+
+
 def main():
     args = parse()
     if args.copy_calibration:
@@ -451,6 +519,9 @@ def main():
     if not args.no_material:
         material, msub, mres = material_leg(max(args.steps // 4, 2), 1, 10000)
         others["material"] = (msub, mres)
+    inter = None
+    if not args.no_intersect:
+        inter, isample = intersect_leg(max(args.steps // 2, 3), 2)
     cpu = None
     errors = None
     if not args.no_cpu:
@@ -459,6 +530,12 @@ def main():
                                          args.cpu_threads, args.cpu_sample, results, others)
         if material is not None:
             material["error_vs_oracle"] = oerr["material"]
+        if inter is not None:
+            inter["cpu_baseline"], inter["oracle_bit_identical_on_sample"] = \
+                intersect_cpu_baseline(*isample, args.cpu_threads)
+            print(f"[bench] intersect cpu baseline {inter['cpu_baseline']['pair_tests_per_s']:.4g}"
+                  f" pair tests/s, bit-identical {inter['oracle_bit_identical_on_sample']}",
+                  file=sys.stderr, flush=True)
         print(f"[bench] cpu baseline {cpu['value']:.4g} tracks/s on {cpu['cores']} threads",
               file=sys.stderr, flush=True)
     out = {
@@ -487,6 +564,7 @@ def main():
         "ragged": ragged,
         "adjoint": adj,
         "material": material,
+        "intersect": inter,
     }
     if "fp64" in rates and "mixed" in rates:
         out["mixed_over_fp64"] = rates["mixed"] / rates["fp64"]

@@ -637,6 +637,28 @@ int kf_set_memory_budget(kf_ctx *ctx, uint64_t bytes) {
   return KF_OK;
 }
 
+int64_t kf_intersect_scratch_bytes(int64_t n_rays) {
+  return n_rays < 0 ? 0 : kfd_intersect_scratch_bytes(n_rays);
+}
+
+int kf_intersect(const kf_polygon_plane *surfaces, int64_t n_surfaces, const double *ray_pos,
+                 const double *ray_dir, int64_t n_rays, int32_t bcheck, kf_intersection *hits,
+                 int64_t capacity, int64_t *n_hits, void *scratch, void *stream) {
+  if (n_surfaces < 0 || n_rays < 0 || capacity < 0)
+    return fail(nullptr, KF_E_INVALID, "kf_intersect: negative size");
+  if (n_surfaces > INT32_MAX || n_rays > INT32_MAX)
+    return fail(nullptr, KF_E_INVALID, "kf_intersect: more than 2^31 - 1 rays or surfaces");
+  if (!n_hits || (n_rays > 0 && (!ray_pos || !ray_dir || !scratch)) ||
+      (n_surfaces > 0 && !surfaces) || (capacity > 0 && !hits))
+    return fail(nullptr, KF_E_INVALID, "kf_intersect: NULL array");
+  const hipError_t e = kfd_launch_intersect(surfaces, n_surfaces, ray_pos, ray_dir, n_rays,
+                                            bcheck, hits, capacity, n_hits, scratch,
+                                            (hipStream_t)stream);
+  if (e != hipSuccess)
+    return fail(nullptr, KF_E_HIP, "kf_intersect: launch failed: %s", hipGetErrorString(e));
+  return KF_OK;
+}
+
 double kf_last_kernel_ms(const kf_ctx *ctx) {
   if (!ctx || !ctx->timed) return -1.0;
   if (hipEventSynchronize(ctx->ev1) != hipSuccess) return -1.0;

@@ -9,13 +9,11 @@
 
 #define KFG_HD __host__ __device__
 #include "kf_gen_track.hpp"
+#include "kf_scan.hpp"
 
 namespace {
 
 constexpr int kSimBlock = 128;
-constexpr int kScanBlock = 256;
-constexpr int kScanPerThread = 4;
-constexpr int kScanTile = kScanBlock * kScanPerThread;
 
 __global__ void __launch_bounds__(kSimBlock)
     kfg_sim_kernel(kfg_config cfg, const kf_plane *planes, kfg::GenField fld, kfg::Philox ph,
@@ -33,68 +31,6 @@ __global__ void __launch_bounds__(kSimBlock)
   if (it >= cfg.n_tracks) return;
   kfg::packTrack(cfg, n_hits, hit_offset, m, su, sv, sg, sigma_u, sigma_v, hit_geo, hit_meas,
                  hit_cov, it);
-}
-
-// Exclusive scan of one tile of kScanTile counts (block-local), tile total to
-// sums[tile].
-__global__ void __launch_bounds__(kScanBlock)
-    kfg_scan_tiles(int64_t n, const int32_t *counts, int64_t *out, int64_t *sums) {
-  __shared__ int64_t part[kScanBlock];
-  const int64_t base = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanPerThread;
-  int64_t v[kScanPerThread];
-  int64_t s = 0;
-#pragma unroll
-  for (int k = 0; k < kScanPerThread; ++k) {
-    v[k] = (base + k < n) ? (int64_t)counts[base + k] : 0;
-    s += v[k];
-  }
-  part[threadIdx.x] = s;
-  __syncthreads();
-  // Hillis-Steele inclusive scan of the per-thread sums
-  for (int w = 1; w < kScanBlock; w <<= 1) {
-    const int64_t add = (threadIdx.x >= (unsigned)w) ? part[threadIdx.x - w] : 0;
-    __syncthreads();
-    part[threadIdx.x] += add;
-    __syncthreads();
-  }
-  int64_t run = part[threadIdx.x] - s;
-#pragma unroll
-  for (int k = 0; k < kScanPerThread; ++k) {
-    if (base + k < n) out[base + k] = run;
-    run += v[k];
-  }
-  if (threadIdx.x == kScanBlock - 1) sums[blockIdx.x] = part[threadIdx.x];
-}
-
-// Exclusive scan of the tile totals in place (one block, a sequential carry
-// over chunks of kScanBlock); out[n] = the grand total.
-__global__ void __launch_bounds__(kScanBlock)
-    kfg_scan_sums(int64_t tiles, int64_t *sums, int64_t n, int64_t *out) {
-  __shared__ int64_t part[kScanBlock];
-  int64_t carry = 0;
-  for (int64_t c0 = 0; c0 < tiles; c0 += kScanBlock) {
-    const int64_t i = c0 + threadIdx.x;
-    const int64_t v = (i < tiles) ? sums[i] : 0;
-    part[threadIdx.x] = v;
-    __syncthreads();
-    for (int w = 1; w < kScanBlock; w <<= 1) {
-      const int64_t add = (threadIdx.x >= (unsigned)w) ? part[threadIdx.x - w] : 0;
-      __syncthreads();
-      part[threadIdx.x] += add;
-      __syncthreads();
-    }
-    if (i < tiles) sums[i] = carry + part[threadIdx.x] - v;
-    const int64_t chunk = part[kScanBlock - 1];
-    __syncthreads();
-    carry += chunk;
-  }
-  if (threadIdx.x == 0) out[n] = carry;
-}
-
-__global__ void __launch_bounds__(kScanBlock)
-    kfg_scan_add(int64_t n, const int64_t *sums, int64_t *out) {
-  const int64_t i = (int64_t)blockIdx.x * kScanBlock + threadIdx.x;
-  if (i < n) out[i] += sums[i / kScanTile];
 }
 
 kfg::GenField genField(const kfg_config *cfg, const kfg_field *field) {
@@ -133,24 +69,14 @@ int kfg_simulate_device(const kfg_config *cfg, const kf_plane *true_planes,
   return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 
-int64_t kfg_offsets_scratch_bytes(int64_t n) {
-  return ((n + kScanTile - 1) / kScanTile + 1) * (int64_t)sizeof(int64_t);
-}
+int64_t kfg_offsets_scratch_bytes(int64_t n) { return kscan::scratchBytes(n); }
 
 int kfg_offsets_device(int64_t n, const int32_t *n_hits, int64_t *hit_offset, void *scratch,
                        void *stream) {
   if (n < 0 || !hit_offset || (n > 0 && (!n_hits || !scratch))) return -1;
-  hipStream_t s = (hipStream_t)stream;
-  int64_t *sums = (int64_t *)scratch;
-  const int64_t tiles = (n + kScanTile - 1) / kScanTile;
-  if (tiles > 0)
-    hipLaunchKernelGGL(kfg_scan_tiles, dim3((unsigned)tiles), dim3(kScanBlock), 0, s, n, n_hits,
-                       hit_offset, sums);
-  hipLaunchKernelGGL(kfg_scan_sums, dim3(1), dim3(kScanBlock), 0, s, tiles, sums, n, hit_offset);
-  if (n > 0)
-    hipLaunchKernelGGL(kfg_scan_add, dim3((unsigned)((n + kScanBlock - 1) / kScanBlock)),
-                       dim3(kScanBlock), 0, s, n, (const int64_t *)sums, hit_offset);
-  return hipGetLastError() == hipSuccess ? 0 : -2;
+  return kscan::exclusiveScan(n, n_hits, hit_offset, scratch, (hipStream_t)stream) == hipSuccess
+             ? 0
+             : -2;
 }
 
 int kfg_pack_hits_device(const kfg_config *cfg, const int32_t *n_hits, const int64_t *hit_offset,

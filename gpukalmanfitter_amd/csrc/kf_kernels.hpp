@@ -133,6 +133,14 @@ extern "C" hipError_t kfd_launch_adjoint_finish(const kfd::FitArgs *args, const 
                                                 double *grad_planes, double *grad_scale,
                                                 double *chi2, hipStream_t stream);
 
+// Batched ray x polygon-plane intersection (kf_intersect.hip).
+extern "C" int64_t kfd_intersect_scratch_bytes(int64_t n_rays);
+extern "C" hipError_t kfd_launch_intersect(const kf_polygon_plane *surfaces, int64_t n_surfaces,
+                                           const double *ray_pos, const double *ray_dir,
+                                           int64_t n_rays, int bcheck, kf_intersection *hits,
+                                           int64_t capacity, int64_t *n_hits, void *scratch,
+                                           hipStream_t s);
+
 // Launch shape of kfd_launch_fit.
 struct KfdLaunch {
   int32_t sort_by_length;  // order positions by decreasing sequence length first

@@ -265,6 +265,48 @@ double kf_last_kernel_ms(const kf_ctx *ctx);
  * Run/GPU/KalmanFitterGPUTest.cu:382-513). */
 int kf_last_phase_ms(const kf_ctx *ctx, double *forward_ms, double *backward_ms);
 
+/* ---- Batched ray x bounded-plane intersection ------------------------------
+ * The reference's intersection test (Run/GPU/IntersectGPUTest.cu:22-48,
+ * 116-172: one ray against every surface of a triangular mesh, then a
+ * compaction of the valid intersections) as one batched call over many rays.
+ *
+ * A plane surface with convex polygon bounds (ConvexPolygonBounds<N>,
+ * include/Surfaces/ConvexPolygonBounds.hpp): 3..KF_POLY_MAX vertices in the
+ * local (u, v) frame, in order around the polygon (the kernel reads a
+ * n_vertices outside 3..KF_POLY_MAX as the nearest bound: the surfaces are
+ * device memory, the host cannot check them). */
+#define KF_POLY_MAX 8
+typedef struct kf_polygon_plane {
+  double center[3];
+  double rot[9];                    /* row-major, columns U, V, normal (as kf_plane) */
+  double vertices[2 * KF_POLY_MAX]; /* (u, v) of vertex i at [2 i], [2 i + 1] */
+  int32_t n_vertices;
+  int32_t pad;
+} kf_polygon_plane;
+
+typedef struct kf_intersection {
+  int32_t ray, surface;
+  double path;        /* path length along the ray direction */
+  double position[3]; /* global position of the intersection */
+} kf_intersection;
+
+/* Intersects every ray with every surface as PlaneSurface::intersect
+ * (PlaneSurface.ipp:239-259, PlanarHelper::intersectionEstimate,
+ * PlanarHelper.hpp:29-48) and keeps what the reference driver keeps: status
+ * reachable (not on the surface within s_onSurfaceTolerance), path >= 0 and,
+ * with bcheck != 0, the local position inside the polygon
+ * (BoundaryCheck::isInsidePolygon with zero tolerance, BoundaryCheck.hpp).
+ * Arrays are device memory: surfaces [n_surfaces], ray_pos / ray_dir
+ * [3][n_rays], hits [capacity], n_hits [1], scratch of
+ * kf_intersect_scratch_bytes(n_rays) bytes.  The kept intersections are
+ * written in (ray, surface) order; *n_hits receives their number, of which
+ * the first min(*n_hits, capacity) are written.  Asynchronous on `stream`.
+ * No kf_ctx: errors are reported by kf_last_error(NULL). */
+int64_t kf_intersect_scratch_bytes(int64_t n_rays);
+int kf_intersect(const kf_polygon_plane *surfaces, int64_t n_surfaces, const double *ray_pos,
+                 const double *ray_dir, int64_t n_rays, int32_t bcheck, kf_intersection *hits,
+                 int64_t capacity, int64_t *n_hits, void *scratch, void *stream);
+
 #ifdef __cplusplus
 }
 #endif
