@@ -390,7 +390,8 @@ def intersect_leg(steps, warmup, n_rays=16384, cpu_rays=2048):
     cap = 64 * n_rays
     out = torch.empty(cap * I.HIT_DTYPE.itemsize, dtype=torch.uint8, device="cuda:0")
     lib = I._typed()
-    scratch = torch.empty(int(lib.kf_intersect_scratch_bytes(n_rays)), dtype=torch.uint8,
+    scratch = torch.empty(int(lib.kf_intersect_scratch_bytes(n_rays, len(surfaces))),
+                          dtype=torch.uint8,
                           device="cuda:0")
     n_hits = torch.zeros(1, dtype=torch.int64, device="cuda:0")
 

@@ -637,8 +637,8 @@ int kf_set_memory_budget(kf_ctx *ctx, uint64_t bytes) {
   return KF_OK;
 }
 
-int64_t kf_intersect_scratch_bytes(int64_t n_rays) {
-  return n_rays < 0 ? 0 : kfd_intersect_scratch_bytes(n_rays);
+int64_t kf_intersect_scratch_bytes(int64_t n_rays, int64_t n_surfaces) {
+  return (n_rays < 0 || n_surfaces < 0) ? 0 : kfd_intersect_scratch_bytes(n_rays, n_surfaces);
 }
 
 int kf_intersect(const kf_polygon_plane *surfaces, int64_t n_surfaces, const double *ray_pos,
@@ -648,7 +648,7 @@ int kf_intersect(const kf_polygon_plane *surfaces, int64_t n_surfaces, const dou
     return fail(nullptr, KF_E_INVALID, "kf_intersect: negative size");
   if (n_surfaces > INT32_MAX || n_rays > INT32_MAX)
     return fail(nullptr, KF_E_INVALID, "kf_intersect: more than 2^31 - 1 rays or surfaces");
-  if (!n_hits || (n_rays > 0 && (!ray_pos || !ray_dir || !scratch)) ||
+  if (!n_hits || !scratch || (n_rays > 0 && (!ray_pos || !ray_dir)) ||
       (n_surfaces > 0 && !surfaces) || (capacity > 0 && !hits))
     return fail(nullptr, KF_E_INVALID, "kf_intersect: NULL array");
   const hipError_t e = kfd_launch_intersect(surfaces, n_surfaces, ray_pos, ray_dir, n_rays,

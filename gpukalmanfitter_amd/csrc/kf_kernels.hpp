@@ -134,7 +134,7 @@ extern "C" hipError_t kfd_launch_adjoint_finish(const kfd::FitArgs *args, const 
                                                 double *chi2, hipStream_t stream);
 
 // Batched ray x polygon-plane intersection (kf_intersect.hip).
-extern "C" int64_t kfd_intersect_scratch_bytes(int64_t n_rays);
+extern "C" int64_t kfd_intersect_scratch_bytes(int64_t n_rays, int64_t n_surfaces);
 extern "C" hipError_t kfd_launch_intersect(const kf_polygon_plane *surfaces, int64_t n_surfaces,
                                            const double *ray_pos, const double *ray_dir,
                                            int64_t n_rays, int bcheck, kf_intersection *hits,

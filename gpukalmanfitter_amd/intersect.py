@@ -30,7 +30,7 @@ def _typed():
     lib = _lib()
     if not getattr(lib, "_kf_ix_typed", False):
         P = C.c_void_p
-        lib.kf_intersect_scratch_bytes.argtypes = [C.c_int64]
+        lib.kf_intersect_scratch_bytes.argtypes = [C.c_int64, C.c_int64]
         lib.kf_intersect_scratch_bytes.restype = C.c_int64
         lib.kf_intersect.argtypes = [P, C.c_int64, P, P, C.c_int64, C.c_int32, P, C.c_int64, P,
                                      P, P]
@@ -144,7 +144,8 @@ class Intersector:
             out = torch.empty(max(capacity, 1) * HIT_DTYPE.itemsize, dtype=torch.uint8,
                               device=self.device)
         if scratch is None:
-            scratch = torch.empty(max(int(lib.kf_intersect_scratch_bytes(n_rays)), 8),
+            scratch = torch.empty(max(int(lib.kf_intersect_scratch_bytes(n_rays,
+                                                                         self.n_surfaces)), 8),
                                   dtype=torch.uint8, device=self.device)
         if n_hits is None:
             n_hits = torch.zeros(1, dtype=torch.int64, device=self.device)

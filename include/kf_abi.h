@@ -298,11 +298,11 @@ typedef struct kf_intersection {
  * (BoundaryCheck::isInsidePolygon with zero tolerance, BoundaryCheck.hpp).
  * Arrays are device memory: surfaces [n_surfaces], ray_pos / ray_dir
  * [3][n_rays], hits [capacity], n_hits [1], scratch of
- * kf_intersect_scratch_bytes(n_rays) bytes.  The kept intersections are
+ * kf_intersect_scratch_bytes(n_rays, n_surfaces) bytes.  The kept intersections are
  * written in (ray, surface) order; *n_hits receives their number, of which
  * the first min(*n_hits, capacity) are written.  Asynchronous on `stream`.
  * No kf_ctx: errors are reported by kf_last_error(NULL). */
-int64_t kf_intersect_scratch_bytes(int64_t n_rays);
+int64_t kf_intersect_scratch_bytes(int64_t n_rays, int64_t n_surfaces);
 int kf_intersect(const kf_polygon_plane *surfaces, int64_t n_surfaces, const double *ray_pos,
                  const double *ray_dir, int64_t n_rays, int32_t bcheck, kf_intersection *hits,
                  int64_t capacity, int64_t *n_hits, void *scratch, void *stream);
