@@ -373,6 +373,17 @@ def material_leg(steps, warmup, sample):
     return out, b.subset(np.arange(min(sample, b.n_tracks))), res
 
 
+def pull_summary(res, batch):
+    """Gaussian fits of the pulls (fitted - truth) / error of the successful
+    tracks, per perigee parameter (the perigeeParamResolution.C report):
+    a physics check of the timed fit's output."""
+    from gpukalmanfitter_amd import writers as W
+    tab = W.params_table(res["params"], res["cov"], res["status"], batch.truth, batch.perigee)
+    rep = W.resolution_report(tab, residual_range="auto")
+    return {k: {"mean": round(v["pull"]["mean"], 4), "sigma": round(v["pull"]["sigma"], 4)}
+            for k, v in rep.items()}
+
+
 def intersect_leg(steps, warmup, n_rays=16384, cpu_rays=2048):
     """Batched ray x triangle intersection (kf_intersect): rays from the
     beam spot against a 10-layer barrel of about 37k triangles (the size of
@@ -504,6 +515,7 @@ def main():
 
     head = precisions[0]
     roof = roofline(head, n, phase_ms[head]) if args.config == 2 else None
+    pulls = pull_summary(results[head], batch)
     ragged = None
     if not args.no_ragged:
         ragged = ragged_leg(max(args.steps // 2, 3), 1)
@@ -557,6 +569,7 @@ def main():
             else "see gpukalmanfitter_amd/configs.py"), "n_tracks": n, "precision": head,
             "overstep_limit_mm": configs.WORKLOAD_FIT_OPTIONS["overstep_limit"]},
         "tracks_per_s": rates,
+        "pulls": pulls,
         "kernel_ms": kernel_ms,
         "h2d_s": h2d_s,
         "roofline": roof,
