@@ -34,7 +34,9 @@ namespace kfd {
 
 constexpr int kIxBlock = 256;                // rays per block (count pass)
 constexpr int kIxSlots = 8;                  // surface indices kept per (ray, chunk)
-constexpr int kIxGroup = 8;                  // sphere tests per group (count pass)
+// sphere tests per group in the count pass: 4 keeps the kernel at 42 VGPRs
+// (8 waves per SIMD hide the scalar-load latency; 8 tests per group need 98)
+constexpr int kIxGroup = 4;
 constexpr int64_t kIxTargetBlocks = 4096;    // count-pass blocks wanted (16 per CU)
 constexpr int64_t kIxMinChunk = 256;         // surfaces per chunk, lower bound
 constexpr int64_t kIxMaxChunk = 8192;        // and upper bound
@@ -126,8 +128,9 @@ __global__ void __launch_bounds__(256)
 
 struct Ray {
   double p[3], d[3];
-  double dd;  // |d|^2
-  double pm;  // margin part of |p|^2
+  double m[3];  // moment p x d
+  double dd;    // |d|^2
+  double pmdd;  // margin part of |p|^2, times |d|^2
 };
 
 KFD_INLINE Ray loadRay(const double *ray_pos, const double *ray_dir, int64_t n_rays, int64_t r) {
@@ -137,20 +140,24 @@ KFD_INLINE Ray loadRay(const double *ray_pos, const double *ray_dir, int64_t n_r
     y.p[k] = ray_pos[(size_t)k * n_rays + r];
     y.d[k] = ray_dir[(size_t)k * n_rays + r];
   }
+  y.m[0] = y.p[1] * y.d[2] - y.p[2] * y.d[1];
+  y.m[1] = y.p[2] * y.d[0] - y.p[0] * y.d[2];
+  y.m[2] = y.p[0] * y.d[1] - y.p[1] * y.d[0];
   y.dd = y.d[0] * y.d[0] + y.d[1] * y.d[1] + y.d[2] * y.d[2];
-  y.pm = kIxAbsMargin * (y.p[0] * y.p[0] + y.p[1] * y.p[1] + y.p[2] * y.p[2]);
+  y.pmdd = kIxAbsMargin * (y.p[0] * y.p[0] + y.p[1] * y.p[1] + y.p[2] * y.p[2]) * y.dd;
   return y;
 }
 
 // The ray's line passes within the sphere (or a component is not finite):
-// |w x d|^2 <= R^2 |d|^2 with w = centre - origin.
+// |(c - p) x d|^2 <= R^2 |d|^2, with (c - p) x d = c x d - p x d.  Only a
+// filter (the margin in R^2 is far above the rounding of either form), so
+// it uses fused multiply-adds: 11 operations per pair.
 KFD_INLINE bool sphereCandidate(const double4 &q, const Ray &y) {
-  const double w0 = q.x - y.p[0], w1 = q.y - y.p[1], w2 = q.z - y.p[2];
-  const double c0 = w1 * y.d[2] - w2 * y.d[1];
-  const double c1 = w2 * y.d[0] - w0 * y.d[2];
-  const double c2 = w0 * y.d[1] - w1 * y.d[0];
-  const double lhs = c0 * c0 + c1 * c1 + c2 * c2;
-  return !(lhs > (q.w + y.pm) * y.dd);
+  const double e0 = fma(q.y, y.d[2], fma(-q.z, y.d[1], -y.m[0]));
+  const double e1 = fma(q.z, y.d[0], fma(-q.x, y.d[2], -y.m[1]));
+  const double e2 = fma(q.x, y.d[1], fma(-q.y, y.d[0], -y.m[2]));
+  const double lhs = fma(e0, e0, fma(e1, e1, e2 * e2));
+  return !(fma(-q.w, y.dd, lhs) > y.pmdd);
 }
 
 // Count pass: block (ray block, chunk); per (ray, chunk) the number of kept
