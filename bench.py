@@ -384,7 +384,7 @@ def pull_summary(res, batch):
             for k, v in rep.items()}
 
 
-def intersect_leg(steps, warmup, n_rays=16384, cpu_rays=2048):
+def intersect_leg(steps, warmup, n_rays=16384, cpu_rays=16384):
     """Batched ray x triangle intersection (kf_intersect): rays from the
     beam spot against a 10-layer barrel of about 37k triangles (the size of
     the reference's TrackML mesh), boundary check on.  Device time from HIP

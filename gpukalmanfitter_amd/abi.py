@@ -5,7 +5,7 @@ calling it is done by :mod:`gpukalmanfitter_amd.fitter`.
 """
 import ctypes as C
 
-KF_ABI_VERSION = 1
+KF_ABI_VERSION = 2
 
 KF_OK = 0
 KF_E_INVALID = -1
@@ -79,6 +79,7 @@ class kf_options(C.Structure):
         ("max_step_size", C.c_double),
         ("target_tolerance", C.c_double),
         ("overstep_limit", C.c_double),
+        ("outlier_chi2_cut", C.c_double),
     ]
 
 
@@ -141,6 +142,7 @@ def default_options(**kw):
     o.max_step_size = 1.7976931348623157e308
     o.target_tolerance = 1e-4
     o.overstep_limit = 0.01
+    o.outlier_chi2_cut = 0.0
     for k, v in kw.items():
         setattr(o, k, v)
     return o

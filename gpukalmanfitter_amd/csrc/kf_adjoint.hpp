@@ -976,6 +976,7 @@ struct TrackAdjoint {
 #pragma unroll
         for (int m = 0; m < 21; ++m) tf.cov[m] = a.start_cov[(size_t)m * a.n + i];
         tf.measured = 0;
+        tf.outliers = 0;
         tf.next = 0;
         tf.start(pv);
         tf.navStatus();

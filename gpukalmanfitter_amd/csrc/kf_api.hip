@@ -118,6 +118,7 @@ void kf_options_default(kf_options *o) {
   o->max_step_size = DBL_MAX;
   o->target_tolerance = 1e-4;
   o->overstep_limit = 0.01;
+  o->outlier_chi2_cut = 0.0;
 }
 
 int kf_ctx_create(int device, kf_ctx **out) {
@@ -267,7 +268,7 @@ int setupArgs(kf_ctx *ctx, const kf_tracks *tr, const kf_options *opt, const cha
   if (opt->precision != KF_PRECISION_FP64 && opt->precision != KF_PRECISION_MIXED)
     return fail(ctx, KF_E_INVALID, "%s: unknown precision %d", who, opt->precision);
   if (opt->max_steps < 0 || opt->max_rk_trials < 0 || opt->max_states < 0 || !(opt->mass > 0.0) ||
-      !(opt->overstep_limit >= 0.0))
+      !(opt->overstep_limit >= 0.0) || !(opt->outlier_chi2_cut >= 0.0))
     return fail(ctx, KF_E_INVALID, "%s: invalid options", who);
   std::memset(&a, 0, sizeof a);
   a.n = tr->n_tracks;
@@ -297,6 +298,7 @@ int setupArgs(kf_ctx *ctx, const kf_tracks *tr, const kf_options *opt, const cha
   a.max_step_size = opt->max_step_size;
   a.target_tolerance = opt->target_tolerance;
   a.overstep_limit = opt->overstep_limit;
+  a.outlier_chi2_cut = opt->outlier_chi2_cut;
   kfd::physicsConstants(a);
   return KF_OK;
 }
@@ -550,6 +552,10 @@ int kf_chi2_gradient(kf_ctx *ctx, const kf_tracks *tr, const kf_options *opt, do
   if (!ctx) return fail(nullptr, KF_E_INVALID, "kf_chi2_gradient: ctx is NULL");
   if (!tr || !opt || !grad_planes || !grad_field_scale)
     return fail(ctx, KF_E_INVALID, "kf_chi2_gradient: NULL tracks/options/outputs");
+  if (opt->outlier_chi2_cut != 0.0)
+    return fail(ctx, KF_E_INVALID,
+                "kf_chi2_gradient: the outlier finder is not differentiated (outlier_chi2_cut "
+                "must be 0)");
   kfd::FitArgs a;
   int rc = setupArgs(ctx, tr, opt, "kf_chi2_gradient", a);
   if (rc != KF_OK) return rc;

@@ -64,6 +64,7 @@ struct FitArgs {
   uint32_t max_rk_trials;
   int32_t max_states;
   double mass, tolerance, step_size_cutoff, max_step_size, target_tolerance, overstep_limit;
+  double outlier_chi2_cut;  // 0: no outlier finder
   // physics constants (host-evaluated exactly as the reference's literals)
   double kK, kMe, kPlasma, kHighland, kHighlandLog, kLandauConv;
   // outputs

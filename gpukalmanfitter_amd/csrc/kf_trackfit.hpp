@@ -60,6 +60,7 @@ struct TrackFit {
 
   // ---- KalmanFitterResult ----
   int measured;
+  int outliers;    // measured states the outlier finder left without an update
   int firstPlane;  // plane-table index of the first measured state
   bool smoothed, finished, result, smootherNaN;
   int64_t h0, h1;
@@ -985,14 +986,23 @@ struct TrackFit {
     const double S10 = cov[sidx(1, 0)] + V01, S11 = cov[sidx(1, 1)] + V11;
     const double det = S00 * S11 - S01 * S10;
     const double i00 = S11 / det, i01 = -S01 / det, i10 = -S10 / det, i11 = S00 / det;
+    const double r0 = m0 - xp[0], r1 = m1 - xp[1];
+    const double chi2 = r0 * (i00 * r0 + i01 * r1) + r1 * (i10 * r0 + i11 * r1);
+    double xf[6];
+    // outlier finder: above the cut the state keeps its prediction (no
+    // update, the covariance stays the predicted one) and stays out of the
+    // chi2 sum and ndf; 0 disables it (the reference's void finder)
+    if (a.outlier_chi2_cut > 0.0 && chi2 > a.outlier_chi2_cut) {
+#pragma unroll
+      for (int r = 0; r < 6; ++r) xf[r] = xp[r];
+      ++outliers;
+    } else {
     double K0[6], K1[6];
 #pragma unroll
     for (int r = 0; r < 6; ++r) {
       K0[r] = cov[sidx(r, 0)] * i00 + cov[sidx(r, 1)] * i10;
       K1[r] = cov[sidx(r, 0)] * i01 + cov[sidx(r, 1)] * i11;
     }
-    const double r0 = m0 - xp[0], r1 = m1 - xp[1];
-    double xf[6];
 #pragma unroll
     for (int r = 0; r < 6; ++r) xf[r] = xp[r] + (K0[r] * r0 + K1[r] * r1);
     // the update in place on cov: rows 0 and 1 of the result go to t01 first
@@ -1044,8 +1054,8 @@ struct TrackFit {
     for (int c = 0; c < 6; ++c) cov[sidx(0, c)] = (S)t01[c];
 #pragma unroll
     for (int c = 1; c < 6; ++c) cov[sidx(1, c)] = (S)t01[5 + c];
-    const double chi2 = r0 * (i00 * r0 + i01 * r1) + r1 * (i10 * r0 + i11 * r1);
     chi2sum = chi2sum + chi2;
+    }
     correctParams(xf);
     storePar(k, KF_SD_XF, xf);
     if (tsf) {
@@ -1456,6 +1466,7 @@ struct TrackFit {
 #pragma unroll
     for (int k = 0; k < 21; ++k) cov[k] = a.start_cov[(size_t)k * a.n + i];
     measured = 0;
+    outliers = 0;
     firstPlane = 0;
     result = true;
     chi2sum = 0.0;
@@ -1485,7 +1496,7 @@ struct TrackFit {
       }
     }
     if (a.out_chi2) a.out_chi2[i] = chi2sum;
-    if (a.out_ndf) a.out_ndf[i] = 2 * measured;
+    if (a.out_ndf) a.out_ndf[i] = 2 * (measured - outliers);
     if (a.out_nmeas) a.out_nmeas[i] = measured;
     if (resume != kResumeNone) {
       saveCarry(resume, steps, terminated);

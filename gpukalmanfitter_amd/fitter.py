@@ -99,6 +99,9 @@ class KalmanFitterOptions:
     target_tolerance: float = 1e-4
     overstep_limit: float = 0.01
     max_states: int = 0
+    outlier_chi2_cut: float = 0.0  # 0 = VoidOutlierFinder
+
+
 
     def to_c(self):
         o = abi.default_options()
@@ -121,6 +124,7 @@ class KalmanFitterOptions:
         o.target_tolerance = self.target_tolerance
         o.overstep_limit = self.overstep_limit
         o.max_states = self.max_states
+        o.outlier_chi2_cut = self.outlier_chi2_cut
         return o
 
 

@@ -107,6 +107,20 @@ class Batch:
             truth=np.ascontiguousarray(self.truth[:, idx]),
         )
 
+    def drop_hits(self, drop):
+        """Batch without the hits where the boolean mask ``drop`` [m] is set:
+        their planes stay in the surface sequences and become holes."""
+        keep = ~np.asarray(drop, dtype=bool)
+        track = np.repeat(np.arange(self.n_tracks), np.diff(self.hit_offset))
+        counts = np.bincount(track[keep], minlength=self.n_tracks)
+        off = np.zeros(self.n_tracks + 1, dtype=np.int64)
+        np.cumsum(counts, out=off[1:])
+        return dataclasses.replace(
+            self, hit_offset=off,
+            hit_geo_id=np.ascontiguousarray(self.hit_geo_id[keep]),
+            hit_meas=np.ascontiguousarray(self.hit_meas[:, keep]),
+            hit_cov=np.ascontiguousarray(self.hit_cov[:, keep]))
+
 
 def _lib():
     lib = load_library("libkfgen.so")

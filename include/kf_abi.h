@@ -58,7 +58,7 @@
 extern "C" {
 #endif
 
-#define KF_ABI_VERSION 1
+#define KF_ABI_VERSION 2
 
 /* return codes */
 #define KF_OK 0
@@ -128,6 +128,15 @@ typedef struct kf_options {
   double overstep_limit;       /* default 0.01 mm: a surface more than this far behind
                                   is unreachable (EigenStepper.hpp:370, hard-coded
                                   m_overstepLimit in the reference) */
+  double outlier_chi2_cut;     /* outlier finder: 0 (default) = VoidOutlierFinder
+                                  (VoidKalmanComponents.hpp:66-80, nothing is an outlier);
+                                  > 0: a measurement whose predicted-residual chi2
+                                  exceeds it is an outlier: no update (filtered =
+                                  predicted), left out of chi2 and ndf, still a
+                                  measurement state (smoothed, counted in n_measured).
+                                  Outliers of a track = n_measured - ndf / 2; a state is
+                                  one when its KF_TS_CHI2 exceeds the cut.  Not
+                                  supported by kf_chi2_gradient. */
 } kf_options;
 
 /* One batch of tracks.  Every pointer is a device pointer for kf_fit(). */
@@ -150,7 +159,7 @@ typedef struct kf_results {
   double *params;     /* [6][n_tracks] fitted parameters on the perigee line */
   double *cov;        /* [21][n_tracks] */
   double *chi2;       /* [n_tracks] sum of predicted-residual chi2 */
-  int32_t *ndf;       /* [n_tracks] 2 * measured (the chi2 includes the start prior) */
+  int32_t *ndf;       /* [n_tracks] 2 * (measured - outliers) */
   int32_t *n_measured;/* [n_tracks] KalmanFitterResult::measurementStates */
   uint32_t *status;   /* [n_tracks] KF_STATUS_* */
   double *track_states; /* optional [max_states][KF_TS_STRIDE][n_tracks] */

@@ -189,6 +189,7 @@ int kfo_fit_batch(const kf_plane *planes, int32_t n_planes, const kfo_field *fie
     fo.multipleScattering = opt->multiple_scattering != 0;
     fo.energyLoss = opt->energy_loss != 0;
     fo.referenceSurface = &line;
+    fo.outlierChi2Cut = opt->outlier_chi2_cut;
     FitResult r;
     const int cap = max_states > 0 ? max_states : count;
     r.states.resize(cap > 0 ? cap : 1);
@@ -209,10 +210,15 @@ int kfo_fit_batch(const kf_plane *planes, int32_t n_planes, const kfo_field *fie
     if (r.smootherNaN) status |= KF_STATUS_SMOOTHER_NAN;
     if (res->status) res->status[it] = status;
     if (res->n_measured) res->n_measured[it] = (int32_t)r.measurementStates;
-    if (res->ndf) res->ndf[it] = 2 * (int32_t)r.measurementStates;
+    if (res->ndf) {
+      int32_t used = 0;
+      for (unsigned k = 0; k < r.measurementStates; ++k) used += r.states[k].outlier ? 0 : 1;
+      res->ndf[it] = 2 * used;
+    }
     if (res->chi2) {
       Real c = 0.0;
-      for (unsigned k = 0; k < r.measurementStates; ++k) c = c + r.states[k].chi2;
+      for (unsigned k = 0; k < r.measurementStates; ++k)
+        if (!r.states[k].outlier) c = c + r.states[k].chi2;
       res->chi2[it] = kval(c);
     }
     if (res->params)
@@ -354,7 +360,7 @@ void kfo_gain_update(const double pred[6], const double pcov[36], const double m
   ts.sl.meas[1] = meas[1];
   for (int a = 0; a < 2; ++a)
     for (int b = 0; b < 2; ++b) ts.sl.cov(a, b) = V[2 * a + b];
-  kf.gainMatrixUpdate(ts, &plane);
+  kf.gainMatrixUpdate(ts, &plane, 0.0);
   for (int k = 0; k < 6; ++k) filt[k] = kval(ts.filtered.par[k]);
   for (int a = 0; a < 6; ++a)
     for (int b = 0; b < 6; ++b) fcov[6 * a + b] = kval(ts.filtered.cov(a, b));

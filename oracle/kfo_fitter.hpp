@@ -591,6 +591,7 @@ struct TrackStateRec {
   M66 jacobian = M66::identity();
   Real pathLength = 0.0;
   Real chi2 = 0.0;
+  bool outlier = false;  // left without an update by the outlier finder
 };
 
 // KalmanFitterResult, include/Fitter/KalmanFitter.hpp:98-120
@@ -613,6 +614,9 @@ struct FitOptions {
   bool multipleScattering = true;
   bool energyLoss = true;
   const Surface *referenceSurface = nullptr;
+  // outlier finder: 0 = VoidOutlierFinder (VoidKalmanComponents.hpp:66-80);
+  // > 0: a predicted-residual chi2 above it leaves the state unupdated
+  double outlierChi2Cut = 0.0;
 };
 
 // ------------------------------------------------------------ the stepper ----
@@ -1005,7 +1009,7 @@ struct KalmanFitter {
   explicit KalmanFitter(const Field *f) : stepper(f) {}
 
   // GainMatrixUpdater::operator(), GainMatrixUpdater.hpp:34-106
-  bool gainMatrixUpdate(TrackStateRec &ts, const Surface *surface) const {
+  bool gainMatrixUpdate(TrackStateRec &ts, const Surface *surface, double outlierCut) const {
     const BoundParams &pred = ts.predicted;
     const M66 &P = pred.cov;
     Mat<2, 6> H = Mat<2, 6>::zero();
@@ -1032,6 +1036,11 @@ struct KalmanFitter {
     // chi2 of the predicted residual (this build's definition)
     ts.chi2 = res[0] * (covInv(0, 0) * res[0] + covInv(0, 1) * res[1]) +
               res[1] * (covInv(1, 0) * res[0] + covInv(1, 1) * res[1]);
+    // the outlier finder's decision: an outlier keeps its prediction
+    if (outlierCut > 0.0 && kval(ts.chi2) > outlierCut) {
+      ts.filtered = makeBound(P, pred.par, surface);
+      ts.outlier = true;
+    }
     return true;
   }
 
@@ -1137,7 +1146,7 @@ struct KalmanFitter {
     ts = TrackStateRec();
     ts.sl = inputs[found];
     stepper.boundState(st.stepping, *surface, ts.predicted, ts.jacobian, ts.pathLength);
-    if (!gainMatrixUpdate(ts, surface)) return false;
+    if (!gainMatrixUpdate(ts, surface, fo.outlierChi2Cut)) return false;
     V8 fp = boundToFree(ts.filtered.par, *ts.filtered.surface);
     stepper.update(st.stepping, fp, ts.filtered.cov);
     materialInteractor(surface, st, fo);
