@@ -48,6 +48,13 @@ enum {
 // dX/dR (9, row-major) of plane seq_first + q
 enum { kPG_C = 0, kPG_R = 3, kPG_N = 12 };
 
+// per-lane LDS slots (doubles, stride KF_BLOCK) for adjoint state of the
+// reverse step loop, so that it does not occupy registers there: the
+// covariance adjoint (carried across the loop, unused in it), the adjoint of
+// the Jacobian's position rows A (constant over a segment's steps), of its
+// direction rows G, and the next G adjoint while a step builds it
+enum { kLDS_CB = 0, kLDS_AB = 25, kLDS_GB = 40, kLDS_GT = 49, kLDS_N = 58 };
+
 struct AdjArgs {
   double *stack;          // [chunk][max_seg_steps][kSR_N][64]
   int32_t max_seg_steps;  // stack depth per lane
@@ -243,12 +250,15 @@ KFD_INLINE void dirColumnVJP(const StepFwd &F, double h, double qop, const doubl
 }
 
 // Backward of one RK step.  In: the adjoints of the step's outputs (position
-// pb, direction db, Jacobian Ab/Gb, and k4b for the propagation derivative of
-// the last step).  Out (in place): the adjoints of its inputs (position,
-// direction, Jacobian), and accumulated hb, qb (q/p) and sb (field scale).
+// pb, direction db, the Jacobian's A and G rows in the LDS slots kLDS_AB and
+// kLDS_GB of `lj`, and k4b for the propagation derivative of the last step).
+// Out (in place): the adjoints of its inputs (position, direction, G rows;
+// the A adjoint passes through a step unchanged), and accumulated hb, qb
+// (q/p) and sb (field scale).  Gin: the J direction rows before the step,
+// element (r, c) at Gin[(3 r + c) * gstride].
 KFD_INLINE void stepVJP(const DevField &fld, const StepFwd &F, const double pos[3],
-                        const double d[3], double h, double qop, const double Gin[3][3],
-                        double pb[3], double db[3], double Gb[3][3], const double Ab[3][5],
+                        const double d[3], double h, double qop, const double *Gin,
+                        uint32_t gstride, double pb[3], double db[3], double *lj,
                         const double k4bIn[3], double &hb, double &qb, double &sb) {
   const double hh = h * 0.5;
   double k1b[3] = {0, 0, 0}, k2b[3] = {0, 0, 0}, k3b[3] = {0, 0, 0};
@@ -257,24 +267,27 @@ KFD_INLINE void stepVJP(const DevField &fld, const StepFwd &F, const double pos[
   double B1b[3] = {0, 0, 0}, Bmb[3] = {0, 0, 0}, Blb[3] = {0, 0, 0};
   double dInb[3] = {0, 0, 0}, posInb[3] = {pb[0], pb[1], pb[2]};
   // ---- J <- D J:  A[:,2+c] += dFdT G[:,c] (+ dFdL for c = 2), G[:,c] = dGdT G[:,c] (+ dGdL),
-  // column by column
-  double GinB[3][3];
-#pragma unroll
+  // column by column (one column's temporaries at a time: not unrolled)
+#pragma unroll 1
   for (int c = 0; c < 3; ++c) {
-    const double g[3] = {Gin[0][c], Gin[1][c], Gin[2][c]};
-    const double aF[3] = {Ab[0][2 + c], Ab[1][2 + c], Ab[2][2 + c]};
-    const double aG[3] = {Gb[0][c], Gb[1][c], Gb[2][c]};
+    double g[3], aF[3], aG[3];
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+      g[r] = Gin[(uint32_t)(3 * r + c) * gstride];
+      aF[r] = lj[(kLDS_AB + 5 * r + 2 + c) * KF_BLOCK];
+      aG[r] = lj[(kLDS_GB + 3 * r + c) * KF_BLOCK];
+    }
     double gb[3] = {0.0, 0.0, 0.0};
     dirColumnVJP(F, h, qop, g, aF, aG, gb, B1b, Bmb, Blb, hb, qb);
 #pragma unroll
-    for (int q = 0; q < 3; ++q) GinB[q][c] = gb[q];
+    for (int q = 0; q < 3; ++q) lj[(kLDS_GT + 3 * q + c) * KF_BLOCK] = gb[q];
   }
   // ---- dFdL = h^2/6 (L1 + L2 + L3), dGdL = h/6 (L1 + 2 L2 + 2 L3 + L4)
   double dFdLb[3], dGdLb[3];
 #pragma unroll
   for (int r = 0; r < 3; ++r) {
-    dFdLb[r] = Ab[r][4];
-    dGdLb[r] = Gb[r][2];
+    dFdLb[r] = lj[(kLDS_AB + 5 * r + 4) * KF_BLOCK];
+    dGdLb[r] = lj[(kLDS_GB + 3 * r + 2) * KF_BLOCK];
   }
   double L1b[3], L2b[3], L3b[3], L4b[3];
 #pragma unroll
@@ -415,9 +428,7 @@ KFD_INLINE void stepVJP(const DevField &fld, const StepFwd &F, const double pos[
     db[k] = dInb[k];
   }
 #pragma unroll
-  for (int q = 0; q < 3; ++q)
-#pragma unroll
-    for (int cc = 0; cc < 3; ++cc) Gb[q][cc] = GinB[q][cc];
+  for (int m = 0; m < 9; ++m) lj[(kLDS_GB + m) * KF_BLOCK] = lj[(kLDS_GT + m) * KF_BLOCK];
 }
 
 // ---------------------------------------------------------------------------
@@ -900,11 +911,12 @@ struct TrackAdjoint {
   const int64_t loc;  // position within the launch
   double *stk;  // this wave's step stack
   uint32_t lane;
+  double *lds;  // this lane's LDS slots: lds[f * KF_BLOCK]
   TrackFit<double, true> tf;  // recomputes segments from the checkpoints
 
   KFD_INLINE TrackAdjoint(const FitArgs &args, const AdjArgs &adj, int64_t idx, int64_t local,
-                          double *ckpt, double *stack, uint32_t ln)
-      : a(args), g(adj), i(idx), loc(local), stk(stack), lane(ln),
+                          double *ckpt, double *stack, uint32_t ln, double *ldsLane)
+      : a(args), g(adj), i(idx), loc(local), stk(stack), lane(ln), lds(ldsLane),
         tf(args, idx, nullptr, nullptr, nullptr, ckpt, ln) {}
 
   KFD_INLINE double *sr(int j, int f) const {
@@ -1090,24 +1102,34 @@ struct TrackAdjoint {
         for (int c = 0; c < 3; ++c) Rb[m][c] += Rb2[m][c];
       }
       addPlane(plane, cb, Rb);
+      // the covariance adjoint waits in LDS while the steps are reversed; the
+      // Jacobian adjoints are kept there during the loop
+#pragma unroll
+      for (int r = 0; r < 5; ++r)
+#pragma unroll
+        for (int c = 0; c < 5; ++c) lds[(kLDS_CB + 5 * r + c) * KF_BLOCK] = Cmb[r][c];
+#pragma unroll
+      for (int r = 0; r < 3; ++r) {
+#pragma unroll
+        for (int c = 0; c < 5; ++c) lds[(kLDS_AB + 5 * r + c) * KF_BLOCK] = Abb[r][c];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) lds[(kLDS_GB + 3 * r + c) * KF_BLOCK] = Gbb[r][c];
+      }
       // ---- RK steps in reverse
       for (int j = nst - 1; j >= 0; --j) {
-        double pin[3], din[3], Gin[3][3];
+        double pin[3], din[3];
 #pragma unroll
         for (int m = 0; m < 3; ++m) {
           pin[m] = *sr(j, kSR_POS + m);
           din[m] = *sr(j, kSR_DIR + m);
         }
-#pragma unroll
-        for (int r = 0; r < 3; ++r)
-#pragma unroll
-          for (int c = 0; c < 3; ++c) Gin[r][c] = *sr(j, kSR_G + 3 * r + c);
         const double h = *sr(j, kSR_H);
         const int tgt = (int)*sr(j, kSR_TGT) - 1;
         StepFwd F;
         F.run(a.field, pin, din, h, qop);
         double hb = 0.0;
-        stepVJP(a.field, F, pin, din, h, qop, Gin, posb, dbb, Gbb, Abb, k4b, hb, qbb, sb);
+        stepVJP(a.field, F, pin, din, h, qop, sr(j, kSR_G), 64u, posb, dbb, lds, k4b, hb, qbb,
+                sb);
 #pragma unroll
         for (int m = 0; m < 3; ++m) k4b[m] = 0.0;
         if (tgt >= 0) {
@@ -1137,14 +1159,14 @@ struct TrackAdjoint {
 #pragma unroll
       for (int r = 0; r < 3; ++r) {
 #pragma unroll
-        for (int c = 0; c < 5; ++c) Ab[r][c] = Abb[r][c];
+        for (int c = 0; c < 5; ++c) Ab[r][c] = lds[(kLDS_AB + 5 * r + c) * KF_BLOCK];
 #pragma unroll
-        for (int c = 0; c < 3; ++c) Gb[r][c] = Gbb[r][c];
+        for (int c = 0; c < 3; ++c) Gb[r][c] = lds[(kLDS_GB + 3 * r + c) * KF_BLOCK];
       }
 #pragma unroll
       for (int r = 0; r < 5; ++r)
 #pragma unroll
-        for (int c = 0; c < 5; ++c) Cb[r][c] = Cmb[r][c];
+        for (int c = 0; c < 5; ++c) Cb[r][c] = lds[(kLDS_CB + 5 * r + c) * KF_BLOCK];
     }
     if (st) {
       for (int q = 0; q < slots; ++q)
