@@ -955,20 +955,12 @@ struct TrackAdjoint {
       if (g.status) g.status[i] = KF_ADJ_SKIPPED;
       return;
     }
-    // adjoint of the state after the update at the surface being processed
+    // adjoint of the state after the update at the surface being processed:
+    // position, direction, q/p in registers; the Jacobian (A and G rows) and
+    // covariance adjoints in this lane's LDS slots, read where consumed
     double pb[3] = {0, 0, 0}, db[3] = {0, 0, 0}, qb = 0.0;
-    double Ab[3][5], Gb[3][3], Cb[5][5];
 #pragma unroll
-    for (int r = 0; r < 3; ++r) {
-#pragma unroll
-      for (int c = 0; c < 5; ++c) Ab[r][c] = 0.0;
-#pragma unroll
-      for (int c = 0; c < 3; ++c) Gb[r][c] = 0.0;
-    }
-#pragma unroll
-    for (int r = 0; r < 5; ++r)
-#pragma unroll
-      for (int c = 0; c < 5; ++c) Cb[r][c] = 0.0;
+    for (int m = 0; m < kLDS_N; ++m) lds[m * KF_BLOCK] = 0.0;
 
     // the stepper charge is set once from the start parameters and never
     // updated (the checkpoints do not carry it)
@@ -1069,7 +1061,17 @@ struct TrackAdjoint {
       // ---- backward at the plane
       double cb[3] = {0, 0, 0}, Rb[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
       double dPre[3] = {0, 0, 0};  // adjoint of the direction at the plane (before the update)
-      jacInitVJP(dEnd, Ab, Gb, dPre, Rb);
+      {
+        double Ab[3][5], Gb[3][3];
+#pragma unroll
+        for (int r = 0; r < 3; ++r) {
+#pragma unroll
+          for (int c = 0; c < 5; ++c) Ab[r][c] = lds[(kLDS_AB + 5 * r + c) * KF_BLOCK];
+#pragma unroll
+          for (int c = 0; c < 3; ++c) Gb[r][c] = lds[(kLDS_GB + 3 * r + c) * KF_BLOCK];
+        }
+        jacInitVJP(dEnd, Ab, Gb, dPre, Rb);
+      }
       // material on this plane, recomputed from the filtered state
       const double pF = fabs(1.0 / P.xf[4]);
       double dF[3];
@@ -1086,14 +1088,23 @@ struct TrackAdjoint {
       // adjoint of the momentum after the plane from that of q/p
       const double pAfterb = -qb * tf.q / (Mt.pOut * Mt.pOut);
       double dbm[3] = {db[0], db[1], db[2]}, nbm[3] = {0, 0, 0};
-      const double pFb = Mt.vjp(a, pl, dF, pAfterb, Cb[2][2], Cb[3][3], Cb[4][4], dbm, nbm);
+      const double pFb = Mt.vjp(a, pl, dF, pAfterb, lds[(kLDS_CB + 12) * KF_BLOCK],
+                                lds[(kLDS_CB + 18) * KF_BLOCK], lds[(kLDS_CB + 24) * KF_BLOCK],
+                                dbm, nbm);
 #pragma unroll
       for (int m = 0; m < 3; ++m) Rb[m][2] += nbm[m];
       double xfb[5] = {0, 0, 0, 0, 0};
       freeFromBoundVJP(P, pb, dbm, pFb, xfb, cb, Rb);
       double posb[3], dbb[3], Abb[3][5], Gbb[3][3], k4b[3], qbb = 0.0, Cmb[5][5], cb2[3],
           Rb2[3][3];
-      P.vjp(dEnd, posEnd, AEnd, k4, xfb, Cb, posb, dbb, Abb, Gbb, k4b, qbb, Cmb, cb2, Rb2);
+      {
+        double Cb[5][5];
+#pragma unroll
+        for (int r = 0; r < 5; ++r)
+#pragma unroll
+          for (int c = 0; c < 5; ++c) Cb[r][c] = lds[(kLDS_CB + 5 * r + c) * KF_BLOCK];
+        P.vjp(dEnd, posEnd, AEnd, k4, xfb, Cb, posb, dbb, Abb, Gbb, k4b, qbb, Cmb, cb2, Rb2);
+      }
 #pragma unroll
       for (int m = 0; m < 3; ++m) {
         cb[m] += cb2[m];
@@ -1156,17 +1167,6 @@ struct TrackAdjoint {
         db[m] = dbb[m];
       }
       qb = qbb;
-#pragma unroll
-      for (int r = 0; r < 3; ++r) {
-#pragma unroll
-        for (int c = 0; c < 5; ++c) Ab[r][c] = lds[(kLDS_AB + 5 * r + c) * KF_BLOCK];
-#pragma unroll
-        for (int c = 0; c < 3; ++c) Gb[r][c] = lds[(kLDS_GB + 3 * r + c) * KF_BLOCK];
-      }
-#pragma unroll
-      for (int r = 0; r < 5; ++r)
-#pragma unroll
-        for (int c = 0; c < 5; ++c) Cb[r][c] = lds[(kLDS_CB + 5 * r + c) * KF_BLOCK];
     }
     if (st) {
       for (int q = 0; q < slots; ++q)
