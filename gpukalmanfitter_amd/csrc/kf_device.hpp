@@ -91,15 +91,22 @@ KFD_INLINE void fieldAt(const DevField &f, double x, double y, double z, double 
     const double ur = f.lo[d] + (double)(i + 1) * f.w[d];
     fr[d] = (v - ll) / (ur - ll);
   }
-  double cz[4][3];
+  // the four (x, y) corner columns: six floats each (the z pair), all
+  // loads issued before any is used, so one memory latency per lookup
+  float v[4][6];
 #pragma unroll
   for (int c = 0; c < 4; ++c) {
     const int ix = idx[0] + (c >> 1), iy = idx[1] + (c & 1);
     const size_t b0 = (((size_t)ix * f.n[1] + iy) * f.n[2] + idx[2]) * 3;
 #pragma unroll
-    for (int k = 0; k < 3; ++k)
-      cz[c][k] = (1.0 - fr[2]) * (double)f.values[b0 + k] + fr[2] * (double)f.values[b0 + 3 + k];
+    for (int k = 0; k < 6; ++k) v[c][k] = f.values[b0 + k];
   }
+  double cz[4][3];
+#pragma unroll
+  for (int c = 0; c < 4; ++c)
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+      cz[c][k] = (1.0 - fr[2]) * (double)v[c][k] + fr[2] * (double)v[c][3 + k];
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
     const double y0 = (1.0 - fr[1]) * cz[0][k] + fr[1] * cz[1][k];
@@ -145,6 +152,17 @@ KFD_INLINE void fieldAtGrad(const DevField &f, const double x[3], double B[3], d
     dfr[d] = clamped ? 0.0 : 1.0 / (ur - ll);
   }
   const double sc = f.scale;
+  // every corner value loaded before any is used (one memory latency):
+  // column (a, b) holds the z pair, six floats
+  float v[4][6];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b) {
+      const size_t o = ((((size_t)(idx[0] + a)) * f.n[1] + (idx[1] + b)) * f.n[2] + idx[2]) * 3;
+#pragma unroll
+      for (int m = 0; m < 6; ++m) v[2 * a + b][m] = f.values[o + m];
+    }
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
     double val[2][2][2];
@@ -153,10 +171,7 @@ KFD_INLINE void fieldAtGrad(const DevField &f, const double x[3], double B[3], d
 #pragma unroll
       for (int b = 0; b < 2; ++b)
 #pragma unroll
-        for (int c = 0; c < 2; ++c) {
-          const size_t o = ((((size_t)(idx[0] + a)) * f.n[1] + (idx[1] + b)) * f.n[2] + (idx[2] + c)) * 3;
-          val[a][b][c] = (double)f.values[o + k];
-        }
+        for (int c = 0; c < 2; ++c) val[a][b][c] = (double)v[2 * a + b][3 * c + k];
     const double wx[2] = {1.0 - fr[0], fr[0]}, wy[2] = {1.0 - fr[1], fr[1]},
                  wz[2] = {1.0 - fr[2], fr[2]};
     double s = 0.0, gx = 0.0, gy = 0.0, gz = 0.0;
