@@ -267,15 +267,23 @@ KFD_INLINE void stepVJP(const DevField &fld, const StepFwd &F, const double pos[
   double B1b[3] = {0, 0, 0}, Bmb[3] = {0, 0, 0}, Blb[3] = {0, 0, 0};
   double dInb[3] = {0, 0, 0}, posInb[3] = {pb[0], pb[1], pb[2]};
   // ---- J <- D J:  A[:,2+c] += dFdT G[:,c] (+ dFdL for c = 2), G[:,c] = dGdT G[:,c] (+ dGdL),
-  // column by column (one column's temporaries at a time: not unrolled)
+  // column by column (one column's temporaries at a time: not unrolled);
+  // the next column of G is loaded while the current one is processed
+  double gn[3];
+#pragma unroll
+  for (int r = 0; r < 3; ++r) gn[r] = Gin[(uint32_t)(3 * r) * gstride];
 #pragma unroll 1
   for (int c = 0; c < 3; ++c) {
     double g[3], aF[3], aG[3];
 #pragma unroll
     for (int r = 0; r < 3; ++r) {
-      g[r] = Gin[(uint32_t)(3 * r + c) * gstride];
+      g[r] = gn[r];
       aF[r] = lj[(kLDS_AB + 5 * r + 2 + c) * KF_BLOCK];
       aG[r] = lj[(kLDS_GB + 3 * r + c) * KF_BLOCK];
+    }
+    if (c < 2) {
+#pragma unroll
+      for (int r = 0; r < 3; ++r) gn[r] = Gin[(uint32_t)(3 * r + c + 1) * gstride];
     }
     double gb[3] = {0.0, 0.0, 0.0};
     dirColumnVJP(F, h, qop, g, aF, aG, gb, B1b, Bmb, Blb, hb, qb);
