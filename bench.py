@@ -373,6 +373,39 @@ def material_leg(steps, warmup, sample):
     return out, b.subset(np.arange(min(sample, b.n_tracks))), res
 
 
+def end_to_end(kf, dev, opts, steps):
+    """The fit as the reference's driver times it (inputs from host memory,
+    results back to host): per step, H2D copies of every input array from
+    pinned host buffers into the resident device buffers, the fit, and D2H
+    copies of every result array into pinned host buffers, on one stream,
+    timed between device synchronisations.  Allocation is outside the timed
+    region (the reference's timing also includes its cudaMalloc/free)."""
+    import torch
+    host_in = {k: v.cpu().pin_memory() for k, v in dev.t.items()}
+    host_out = {k: torch.empty(v.shape, dtype=v.dtype).pin_memory() for k, v in dev.out.items()}
+    h2d = sum(v.numel() * v.element_size() for v in host_in.values())
+    d2h = sum(v.numel() * v.element_size() for v in host_out.values())
+
+    def step():
+        for k, v in host_in.items():
+            dev.t[k].copy_(v, non_blocking=True)
+        kf.fit_batch(dev, opts)
+        for k, v in host_out.items():
+            v.copy_(dev.out[k], non_blocking=True)
+
+    step()
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for _ in range(steps):
+        step()
+    torch.cuda.synchronize()
+    sec = (time.perf_counter() - t) / steps
+    return {"tracks_per_s": dev.n_tracks / sec, "ms_per_step": sec * 1e3, "h2d_bytes": h2d,
+            "d2h_bytes": d2h, "pcie_gb_per_s": (h2d + d2h) / sec / 1e9,
+            "note": "pinned host inputs -> H2D -> fit -> D2H results per step; not the "
+                    "headline value (inputs resident in HBM)"}
+
+
 def pull_summary(res, batch):
     """Gaussian fits of the pulls (fitted - truth) / error of the successful
     tracks, per perigee parameter (the perigeeParamResolution.C report):
@@ -516,6 +549,11 @@ def main():
     head = precisions[0]
     roof = roofline(head, n, phase_ms[head]) if args.config == 2 else None
     pulls = pull_summary(results[head], batch)
+    e2e = end_to_end(kf, dev, KalmanFitterOptions(precision=head, **configs.WORKLOAD_FIT_OPTIONS),
+                     max(args.steps // 2, 3))
+    print(f"[bench] end to end ({head}, H2D + fit + D2H): {e2e['tracks_per_s']:.4g} tracks/s, "
+          f"{e2e['ms_per_step']:.2f} ms/step, {e2e['pcie_gb_per_s']:.1f} GB/s over PCIe",
+          file=sys.stderr, flush=True)
     ragged = None
     if not args.no_ragged:
         ragged = ragged_leg(max(args.steps // 2, 3), 1)
@@ -570,6 +608,7 @@ def main():
             "overstep_limit_mm": configs.WORKLOAD_FIT_OPTIONS["overstep_limit"]},
         "tracks_per_s": rates,
         "pulls": pulls,
+        "end_to_end": e2e,
         "kernel_ms": kernel_ms,
         "h2d_s": h2d_s,
         "roofline": roof,
