@@ -24,6 +24,10 @@
 //   -p,--precision fp64|mixed
 //   -n,--repeat N          timed fits (default 10, the median is reported)
 //   --reference-overstep   keep the reference's 0.01 mm overstep limit
+//   --with-copies          time each fit with its inputs copied in from pinned
+//                          host memory and its results copied out, every
+//                          stream's chunk in its own stream (the reference
+//                          driver's timing also covers the transfers)
 //   --per-track K          also fit the first K tracks one at a time through
 //                          kfmi::KalmanFitter::fit (the reference's per-track
 //                          call) and report their largest difference from
@@ -36,6 +40,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <type_traits>
 #include <fstream>
 #include <memory>
 #include <string>
@@ -129,7 +134,7 @@ void usage() {
                "\t-g,--grid-size X*Y\n\t-b,--block-size X*Y*Z\n\t-s,--shared-memory 0\n"
                "\t-m,--smoothing 0|1\n\t-a,--machine NAME\n\t-c,--config 1|2|3|5\n"
                "\t-p,--precision fp64|mixed\n\t-n,--repeat N\n\t--reference-overstep\n"
-               "\t--per-track K\n"
+               "\t--with-copies\n\t--per-track K\n"
                "\t--print-config (prints the generator settings and exits)\n");
 }
 
@@ -165,6 +170,7 @@ int main(int argc, char **argv) {
   int nStreams = 1, nThreads = 0, config = 1, repeat = 10;
   int64_t perTrack = 0;
   bool output = false, smoothing = true, refOverstep = false, printConfig = false;
+  bool withCopies = false;
   std::string device = "gpu", machine, precision = "fp64";
   for (int i = 1; i < argc; ++i) {
     const std::string a = argv[i];
@@ -213,6 +219,8 @@ int main(int argc, char **argv) {
       repeat = std::max(1, std::atoi(next("--repeat").c_str()));
     } else if (a == "--reference-overstep") {
       refOverstep = true;
+    } else if (a == "--with-copies") {
+      withCopies = true;
     } else if (a == "--per-track") {
       perTrack = std::atoll(next("--per-track").c_str());
     } else if (a == "--print-config") {
@@ -377,9 +385,103 @@ int main(int argc, char **argv) {
   opt.propagator.precision = (precision == "mixed") ? KF_PRECISION_MIXED : KF_PRECISION_FP64;
   if (!refOverstep) opt.propagator.overstep_limit = 1.0;
   opt.propagator.max_states = maxp;
+  // --with-copies: pinned host images of every input and output; each
+  // stream's chunk copies its own ranges (per-track arrays in the chunk's
+  // [component][count] blocks, its hit range of every hit component)
+  struct HostChunk {
+    double *sp = nullptr, *sc = nullptr, *pg = nullptr, *par = nullptr, *cov = nullptr,
+           *chi = nullptr;
+    int64_t *off = nullptr;
+    int32_t *ndf = nullptr, *nm = nullptr;
+    uint32_t *st = nullptr;
+  };
+  std::vector<HostChunk> hchunks(nStreams);
+  int32_t *hsf = nullptr, *hsn = nullptr, *hhg = nullptr;
+  double *hhm = nullptr, *hhc = nullptr;
+  bool copyFailed = false;
+  auto pinned = [&](auto *&ptr, size_t count, const auto *src) {
+    using T = std::remove_reference_t<decltype(*ptr)>;
+    void *q = nullptr;
+    if (hipHostMalloc(&q, (count ? count : 1) * sizeof(T)) != hipSuccess) {
+      copyFailed = true;
+      return;
+    }
+    ptr = (T *)q;
+    if (src) std::memcpy(ptr, src, count * sizeof(T));
+  };
+  if (withCopies) {
+    pinned(hsf, (size_t)n, sf.data());
+    pinned(hsn, (size_t)n, scount.data());
+    pinned(hhg, (size_t)m, hg.data());
+    pinned(hhm, (size_t)2 * m, hm.data());
+    pinned(hhc, (size_t)3 * m, hc.data());
+    for (int s = 0; s < nStreams && !copyFailed; ++s) {
+      const Chunk &c = chunks[s];
+      HostChunk &h = hchunks[s];
+      const size_t k = (size_t)c.count;
+      pinned(h.sp, 6 * k, (const double *)nullptr);
+      pinned(h.sc, 21 * k, (const double *)nullptr);
+      pinned(h.pg, 3 * k, (const double *)nullptr);
+      pinned(h.off, k + 1, off.data() + c.first);
+      pinned(h.par, 6 * k, (const double *)nullptr);
+      pinned(h.cov, 21 * k, (const double *)nullptr);
+      pinned(h.chi, k, (const double *)nullptr);
+      pinned(h.ndf, k, (const int32_t *)nullptr);
+      pinned(h.nm, k, (const int32_t *)nullptr);
+      pinned(h.st, k, (const uint32_t *)nullptr);
+      if (copyFailed) break;
+      for (int comp = 0; comp < 6; ++comp)
+        std::memcpy(h.sp + comp * k, sp.data() + (size_t)comp * n + c.first, k * sizeof(double));
+      for (int comp = 0; comp < 21; ++comp)
+        std::memcpy(h.sc + comp * k, sc.data() + (size_t)comp * n + c.first, k * sizeof(double));
+      for (int comp = 0; comp < 3; ++comp)
+        std::memcpy(h.pg + comp * k, pg.data() + (size_t)comp * n + c.first, k * sizeof(double));
+    }
+    if (copyFailed) {
+      std::fprintf(stderr, "ERROR: hipHostMalloc failed (--with-copies)\n");
+      return 1;
+    }
+  }
+  auto copyIn = [&](int s) {
+    const Chunk &c = chunks[s];
+    const HostChunk &h = hchunks[s];
+    const size_t k = (size_t)c.count;
+    const int64_t h0 = off[c.first], nh = off[c.first + c.count] - h0;
+    auto cp = [&](void *d, const void *src, size_t bytes) {
+      if (bytes && hipMemcpyAsync(d, src, bytes, hipMemcpyHostToDevice, c.stream) != hipSuccess)
+        copyFailed = true;
+    };
+    cp(c.sp, h.sp, 6 * k * sizeof(double));
+    cp(c.sc, h.sc, 21 * k * sizeof(double));
+    cp(c.pg, h.pg, 3 * k * sizeof(double));
+    cp(c.off, h.off, (k + 1) * sizeof(int64_t));
+    cp(dsf + c.first, hsf + c.first, k * sizeof(int32_t));
+    cp(dsn + c.first, hsn + c.first, k * sizeof(int32_t));
+    cp(dhg + h0, hhg + h0, nh * sizeof(int32_t));
+    for (int comp = 0; comp < 2; ++comp)
+      cp(dhm + comp * m + h0, hhm + comp * m + h0, nh * sizeof(double));
+    for (int comp = 0; comp < 3; ++comp)
+      cp(dhc + comp * m + h0, hhc + comp * m + h0, nh * sizeof(double));
+  };
+  auto copyOut = [&](int s) {
+    const Chunk &c = chunks[s];
+    const HostChunk &h = hchunks[s];
+    const size_t k = (size_t)c.count;
+    auto cp = [&](void *dst, const void *d, size_t bytes) {
+      if (bytes && hipMemcpyAsync(dst, d, bytes, hipMemcpyDeviceToHost, c.stream) != hipSuccess)
+        copyFailed = true;
+    };
+    cp(h.par, c.par, 6 * k * sizeof(double));
+    cp(h.cov, c.cov, 21 * k * sizeof(double));
+    cp(h.chi, c.chi, k * sizeof(double));
+    cp(h.ndf, c.ndf, k * sizeof(int32_t));
+    cp(h.nm, c.nm, k * sizeof(int32_t));
+    cp(h.st, c.st, k * sizeof(uint32_t));
+  };
   auto fitAll = [&]() {
     for (int s = 0; s < nStreams; ++s) {
       Chunk &c = chunks[s];
+      if (withCopies) copyIn(s);
       kf_tracks t{};
       t.n_tracks = c.count;
       t.n_hits = m;  // hit arrays are shared; offsets index the full arrays
@@ -400,6 +502,7 @@ int main(int argc, char **argv) {
       r.n_measured = c.nm;
       r.status = c.st;
       fitters[s]->fitBatch(t, opt, r, c.stream);
+      if (withCopies) copyOut(s);
     }
     for (int s = 0; s < nStreams; ++s) (void)hipStreamSynchronize(chunks[s].stream);
   };
@@ -414,8 +517,23 @@ int main(int argc, char **argv) {
   std::sort(ms.begin(), ms.end());
   const double med = ms[ms.size() / 2];
   std::printf("INFO: Time (ms) for KF track fitting on device 0 : %f\n", med);
-  std::printf("INFO: %s %.4g tracks/s (median of %d fits, %d stream(s))\n",
-              machine.empty() ? "MI355X" : machine.c_str(), n / (med * 1e-3), repeat, nStreams);
+  std::printf("INFO: %s %.4g tracks/s (median of %d fits, %d stream(s)%s)\n",
+              machine.empty() ? "MI355X" : machine.c_str(), n / (med * 1e-3), repeat, nStreams,
+              withCopies ? ", inputs copied in and results out in the timed region" : "");
+  if (copyFailed) {
+    std::fprintf(stderr, "ERROR: hipMemcpyAsync failed (--with-copies)\n");
+    return 1;
+  }
+  if (withCopies) {
+    // the copied-out results must equal the device results
+    int64_t diff = 0;
+    std::vector<uint32_t> stDev(n);
+    toHost(stDev, dst);
+    for (int s = 0; s < nStreams; ++s)
+      for (int64_t j = 0; j < chunks[s].count; ++j)
+        diff += (hchunks[s].st[j] != stDev[chunks[s].first + j]) ? 1 : 0;
+    std::printf("INFO: copied-out statuses differing from the device's: %lld\n", (long long)diff);
+  }
 
   // ---- results ----
   std::vector<double> par(6 * n);
