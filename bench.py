@@ -62,8 +62,8 @@ HBM_PEAK_GBS = 8000.0
 # calibration (profiles/kf_fit_config2_1M_tracks.md).
 TRAFFIC_TRACKS = 1048576
 TRAFFIC_BYTES_PER_LAUNCH = {
-    ("fp64", "forward"): 4.786e9 + 5.662e9, ("fp64", "backward"): 5.341e9 + 0.231e9,
-    ("mixed", "forward"): 4.595e9 + 3.481e9, ("mixed", "backward"): 3.344e9 + 0.231e9,
+    ("fp64", "forward"): 4.747e9 + 5.662e9, ("fp64", "backward"): 5.343e9 + 0.231e9,
+    ("mixed", "forward"): 4.502e9 + 3.481e9, ("mixed", "backward"): 3.344e9 + 0.231e9,
 }
 PARAM_NAMES = ("loc0", "loc1", "phi", "theta", "qop", "t")
 
