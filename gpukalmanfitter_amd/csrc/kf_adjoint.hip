@@ -9,13 +9,14 @@ namespace kfd {
 
 __global__ void __launch_bounds__(KF_BLOCK) kf_adjoint_kernel(FitArgs args, AdjArgs adj) {
   __shared__ double lds[kLDS_N * KF_BLOCK];
+  __shared__ float ldf[kLDSF_N * KF_BLOCK];
   const int64_t local = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const size_t chunk = __builtin_amdgcn_readfirstlane((uint32_t)(local >> 6));
   double *ck = args.tsc + chunk * (size_t)args.max_states * KF_CK_N * 64u;
   double *stk = adj.stack + chunk * (size_t)adj.max_seg_steps * kSR_N * 64u;
   if (local >= args.count) return;
   TrackAdjoint ta(args, adj, args.first + local, local, ck, stk, (uint32_t)(local & 63),
-                  lds + threadIdx.x);
+                  lds + threadIdx.x, ldf + threadIdx.x);
   ta.run();
 }
 

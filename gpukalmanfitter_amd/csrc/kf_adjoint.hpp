@@ -54,6 +54,10 @@ enum { kPG_C = 0, kPG_R = 3, kPG_N = 12 };
 // the Jacobian's position rows A (constant over a segment's steps), of its
 // direction rows G, and the next G adjoint while a step builds it
 enum { kLDS_CB = 0, kLDS_AB = 25, kLDS_GB = 40, kLDS_GT = 49, kLDS_N = 58 };
+// float slots: the grid-field gradient at a reversed step's three stage
+// points (27); with the double slots 73.2 KB per 128-lane block, two blocks
+// (four waves) per CU
+enum { kLDSF_N = 27 };
 
 struct AdjArgs {
   double *stack;          // [chunk][max_seg_steps][kSR_N][64]
@@ -96,6 +100,19 @@ KFD_INLINE void fieldVJP(const DevField &fld, const double x[3], const double B[
   }
 }
 
+// Grid field: the backward of a lookup whose gradient the forward recompute
+// stored (dB[k][a] of stage point `pt` at dBs[(9 pt + 3 k + a) * KF_BLOCK],
+// float), instead of reading the corners again.
+KFD_INLINE void fieldVJPStored(const DevField &fld, const float *dBs, int pt, const double B[3],
+                               const double Bb[3], double xb[3], double &sb) {
+  sb += (Bb[0] * B[0] + Bb[1] * B[1] + Bb[2] * B[2]) / fld.scale;
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+    xb[a] += Bb[0] * (double)dBs[(9 * pt + a) * KF_BLOCK] +
+             Bb[1] * (double)dBs[(9 * pt + 3 + a) * KF_BLOCK] +
+             Bb[2] * (double)dBs[(9 * pt + 6 + a) * KF_BLOCK];
+}
+
 // One RK4 step (EigenStepper.ipp:226-332 at its accepted length) with every
 // intermediate the backward step needs.
 struct StepFwd {
@@ -106,16 +123,33 @@ struct StepFwd {
   double L1[3], L2[3], L3[3], L4[3];
   double du[3], dun;
 
+  // dBs: for a grid field, the field gradient at the three stage points is
+  // stored there (one lookup gives the value and the gradient); nullptr: the
+  // backward reads the corners again
+  KFD_INLINE void lookup(const DevField &fld, const double x[3], double B[3], float *dBs,
+                         int pt) {
+    if (dBs && fld.grid) {
+      double dB[3][3];
+      fieldAtGrad(fld, x, B, dB);
+#pragma unroll
+      for (int k = 0; k < 3; ++k)
+#pragma unroll
+        for (int a = 0; a < 3; ++a) dBs[(9 * pt + 3 * k + a) * KF_BLOCK] = (float)dB[k][a];
+    } else {
+      fieldAt(fld, x[0], x[1], x[2], B);
+    }
+  }
+
   KFD_INLINE void run(const DevField &fld, const double pos[3], const double d[3], double h,
-                      double qop) {
+                      double qop, float *dBs = nullptr) {
     const double hh = h * 0.5;
-    fieldAt(fld, pos[0], pos[1], pos[2], B1);
+    lookup(fld, pos, B1, dBs, 0);
     crs(d, B1, k1);
 #pragma unroll
     for (int k = 0; k < 3; ++k) k1[k] *= qop;
 #pragma unroll
     for (int k = 0; k < 3; ++k) p1[k] = pos[k] + hh * d[k] + (h * h * 0.125) * k1[k];
-    fieldAt(fld, p1[0], p1[1], p1[2], Bm);
+    lookup(fld, p1, Bm, dBs, 1);
 #pragma unroll
     for (int k = 0; k < 3; ++k) t2[k] = d[k] + hh * k1[k];
     crs(t2, Bm, k2);
@@ -128,7 +162,7 @@ struct StepFwd {
     for (int k = 0; k < 3; ++k) k3[k] *= qop;
 #pragma unroll
     for (int k = 0; k < 3; ++k) p2[k] = pos[k] + h * d[k] + (h * h * 0.5) * k3[k];
-    fieldAt(fld, p2[0], p2[1], p2[2], Bl);
+    lookup(fld, p2, Bl, dBs, 2);
 #pragma unroll
     for (int k = 0; k < 3; ++k) t4[k] = d[k] + h * k3[k];
     crs(t4, Bl, k4);
@@ -259,7 +293,8 @@ KFD_INLINE void dirColumnVJP(const StepFwd &F, double h, double qop, const doubl
 KFD_INLINE void stepVJP(const DevField &fld, const StepFwd &F, const double pos[3],
                         const double d[3], double h, double qop, const double *Gin,
                         uint32_t gstride, double pb[3], double db[3], double *lj,
-                        const double k4bIn[3], double &hb, double &qb, double &sb) {
+                        const double k4bIn[3], double &hb, double &qb, double &sb,
+                        const float *dBs = nullptr) {
   const double hh = h * 0.5;
   double k1b[3] = {0, 0, 0}, k2b[3] = {0, 0, 0}, k3b[3] = {0, 0, 0};
   double k4b[3] = {k4bIn[0], k4bIn[1], k4bIn[2]};
@@ -376,7 +411,10 @@ KFD_INLINE void stepVJP(const DevField &fld, const StepFwd &F, const double pos[
   }
   // Bl = s F(p2)
   double p2b[3] = {0, 0, 0};
-  fieldVJP(fld, F.p2, F.Bl, Blb, p2b, sb);
+  if (dBs && fld.grid)
+    fieldVJPStored(fld, dBs, 2, F.Bl, Blb, p2b, sb);
+  else
+    fieldVJP(fld, F.p2, F.Bl, Blb, p2b, sb);
   // p2 = pos + h d + h^2/2 k3
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
@@ -413,7 +451,10 @@ KFD_INLINE void stepVJP(const DevField &fld, const StepFwd &F, const double pos[
   }
   // Bm = s F(p1)
   double p1b[3] = {0, 0, 0};
-  fieldVJP(fld, F.p1, F.Bm, Bmb, p1b, sb);
+  if (dBs && fld.grid)
+    fieldVJPStored(fld, dBs, 1, F.Bm, Bmb, p1b, sb);
+  else
+    fieldVJP(fld, F.p1, F.Bm, Bmb, p1b, sb);
   // p1 = pos + hh d + h^2/8 k1
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
@@ -429,7 +470,10 @@ KFD_INLINE void stepVJP(const DevField &fld, const StepFwd &F, const double pos[
   for (int k = 0; k < 3; ++k) w[k] = qop * k1b[k];
   crsVJP(d, F.B1, w, dInb, B1b);
   // B1 = s F(pos)
-  fieldVJP(fld, pos, F.B1, B1b, posInb, sb);
+  if (dBs && fld.grid)
+    fieldVJPStored(fld, dBs, 0, F.B1, B1b, posInb, sb);
+  else
+    fieldVJP(fld, pos, F.B1, B1b, posInb, sb);
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
     pb[k] = posInb[k];
@@ -920,11 +964,13 @@ struct TrackAdjoint {
   double *stk;  // this wave's step stack
   uint32_t lane;
   double *lds;  // this lane's LDS slots: lds[f * KF_BLOCK]
+  float *ldf;   // this lane's float LDS slots: the grid-field gradient of a reversed step
   TrackFit<double, true> tf;  // recomputes segments from the checkpoints
 
   KFD_INLINE TrackAdjoint(const FitArgs &args, const AdjArgs &adj, int64_t idx, int64_t local,
-                          double *ckpt, double *stack, uint32_t ln, double *ldsLane)
-      : a(args), g(adj), i(idx), loc(local), stk(stack), lane(ln), lds(ldsLane),
+                          double *ckpt, double *stack, uint32_t ln, double *ldsLane,
+                          float *ldfLane)
+      : a(args), g(adj), i(idx), loc(local), stk(stack), lane(ln), lds(ldsLane), ldf(ldfLane),
         tf(args, idx, nullptr, nullptr, nullptr, ckpt, ln) {}
 
   KFD_INLINE double *sr(int j, int f) const {
@@ -1145,10 +1191,10 @@ struct TrackAdjoint {
         const double h = *sr(j, kSR_H);
         const int tgt = (int)*sr(j, kSR_TGT) - 1;
         StepFwd F;
-        F.run(a.field, pin, din, h, qop);
+        F.run(a.field, pin, din, h, qop, ldf);
         double hb = 0.0;
         stepVJP(a.field, F, pin, din, h, qop, sr(j, kSR_G), 64u, posb, dbb, lds, k4b, hb, qbb,
-                sb);
+                sb, ldf);
 #pragma unroll
         for (int m = 0; m < 3; ++m) k4b[m] = 0.0;
         if (tgt >= 0) {

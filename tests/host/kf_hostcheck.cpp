@@ -149,10 +149,12 @@ int kfh_chi2_gradient(const kf_plane *planes, int32_t n_planes, const kfh_field 
   g.scale_adj = sa.data();
   g.status = flags;
   std::vector<double> lds((size_t)kLDS_N * KF_BLOCK);  // one lane's slots at a time
+  std::vector<float> ldf((size_t)kLDSF_N * KF_BLOCK);
   for (int64_t i = 0; i < n; ++i) {
     const size_t ch = (size_t)(i >> 6);
     TrackAdjoint ta(h.a, g, i, i, h.a.tsc + ch * h.per * KF_CK_N,
-                    stack.data() + ch * (size_t)cap * kSR_N * 64, (uint32_t)(i & 63), lds.data());
+                    stack.data() + ch * (size_t)cap * kSR_N * 64, (uint32_t)(i & 63), lds.data(),
+                    ldf.data());
     ta.run();
   }
   double chi = 0.0, s = 0.0;
