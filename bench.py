@@ -491,7 +491,8 @@ def main():
         return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (got --gpus {args.gpus}); "
-              "tracks are independent, see DESIGN.md for how the batch would shard", file=sys.stderr)
+              "tracks are independent: give each GPU's process a contiguous range of tracks",
+              file=sys.stderr)
         sys.exit(2)
     import numpy as np
     import torch
