@@ -86,6 +86,17 @@ struct TrackRng {
     const double u1 = uniform(), u2 = uniform();
     return sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2);
   }
+  // Standard Landau variate (density (1/pi) int_0^inf exp(-t ln t - x t)
+  // sin(pi t) dt, the convention of CERNLIB's RANLAN): the totally skewed
+  // stable law of index 1 by the method of Chambers, Mallows and Stuck
+  // (J. Amer. Statist. Assoc. 71 (1976) 340): V uniform in (-pi/2, pi/2),
+  // W exponential, X = (2/pi) [(pi/2 + V) tan V - ln((pi/2) W cos V / (pi/2 + V))].
+  KFG_HD double landau() {
+    const double hp = 1.5707963267948966;
+    const double V = 3.141592653589793 * (uniform() - 0.5);
+    const double W = -log(uniform());
+    return ((hp + V) * tan(V) - log(hp * W * cos(V) / (hp + V))) / hp;
+  }
 };
 
 // ------------------------------------------------------------------ field ----
@@ -196,13 +207,17 @@ KFG_HD inline bool propagateToPlane(const GenField &fld, double qop, double pos[
   return false;
 }
 
-// RPP formulas for the material simulation (same physics the fit models:
-// Bethe mean loss, Highland width, Landau-width Gaussian straggling).
+// RPP formulas for the material simulation, as the reference's simulation
+// (ActsFatras BetheBloch and Highland): the energy loss is drawn from a
+// Landau distribution whose location is the most probable loss
+// (computeEnergyLossLandau, Interactions.ipp:220-240, RPP eq. 33.11) and
+// whose scale is the Gaussian-equivalent width of the Landau FWHM 4 eps
+// (computeEnergyLossLandauSigma, :298-314); Highland scattering width.
 struct GenMaterial {
   double thickness, x0, ar, z, molarRho;
 };
 KFG_HD inline void materialEffects(const GenMaterial &m, double mass, double p, double pathCorr,
-                                   double *meanLoss, double *sigmaE, double *theta0) {
+                                   double *mpvLoss, double *sigmaE, double *theta0) {
   const double kMe = 0.5109989461e-3;
   const double kK = 0.307075e-3 * 100.0;
   const double x = m.thickness * pathCorr;
@@ -218,7 +233,12 @@ KFG_HD inline void materialEffects(const GenMaterial &m, double mass, double p, 
   const double wmax = 2 * kMe * bg * bg / (1.0 + 2 * gamma * mfrac + mfrac * mfrac);
   double dhalf = 0;
   if (bg >= 10.0) dhalf = log(bg) + log(28.816e-9 * sqrt(Ne) / I) - 0.5;
-  *meanLoss = eps * (0.5 * log(u / I) + 0.5 * log(wmax / I) - beta2 - dhalf);
+  (void)u;
+  (void)wmax;
+  // the reference's most probable value takes the particle's mass in its
+  // mass term (Interactions.ipp:235; RPP eq. 33.11 has the electron's)
+  const double tMass = 2 * mass * bg * bg;
+  *mpvLoss = eps * (log(tMass / I) + log(eps / I) + 0.2 - beta2 - 2 * dhalf);
   *sigmaE = 4 * eps / (2 * sqrt(2 * log(2.0)));
   const double t = sqrt(x / m.x0 * q2OverBeta2);
   *theta0 = 13.6e-3 / p * t * (1.0 + 0.038 * 2 * log(t));
@@ -287,10 +307,10 @@ KFG_HD inline void simulateTrack(const kfg_config &cfg, const kf_plane *true_pla
     if (cfg.material && pl.thickness > 0.0) {
       const GenMaterial m{pl.thickness, pl.x0, pl.ar, pl.z, pl.molar_rho};
       const double pc = 1.0 / fabs(dot3(nrm, dir));
-      double meanLoss, sigmaE, theta0;
-      materialEffects(m, cfg.mass, pp, pc, &meanLoss, &sigmaE, &theta0);
-      double loss = meanLoss + sigmaE * rs.normal();
-      if (loss < 0) loss = 0;
+      double mpvLoss, sigmaE, theta0;
+      materialEffects(m, cfg.mass, pp, pc, &mpvLoss, &sigmaE, &theta0);
+      double loss = mpvLoss + sigmaE * rs.landau();  // Landau(location, scale)
+      if (loss < 0) loss = 0;  // the Landau's left tail, rare at these widths
       const double E = sqrt(cfg.mass * cfg.mass + pp * pp) - loss;
       if (E <= cfg.mass * 1.0001) break;  // stopped
       pp = sqrt(E * E - cfg.mass * cfg.mass);

@@ -134,6 +134,7 @@ def _lib():
                                       P, P, P, C.c_int32]
         lib.kfg_fill_grid.argtypes = [P, C.c_int32, C.c_int32, C.c_int32, P, P, C.c_double, C.c_double]
         lib.kfg_philox.argtypes = [P, P]
+        lib.kfg_landau.argtypes = [C.c_uint64, C.c_int64, P]
         lib._kfg_typed = True
     return lib
 
@@ -235,6 +236,14 @@ def philox4x32_10(counter, key):
     k = np.array(key, dtype=np.uint32)
     _lib().kfg_philox(_ptr(c), _ptr(k))
     return [int(x) for x in c]
+
+
+def landau_samples(n, seed=1):
+    """n standard Landau variates of the generator's sampler (the energy-loss
+    straggling of the material simulation), for its distribution test."""
+    out = np.empty(int(n), dtype=np.float64)
+    _lib().kfg_landau(C.c_uint64(seed), int(n), _ptr(out))
+    return out
 
 
 def _gpu_lib():

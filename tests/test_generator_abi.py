@@ -109,3 +109,19 @@ def test_abi_rejects_null_arguments_before_any_device_work(built):
     assert lib.kf_fit(None, None, None, None, None) == abi.KF_E_INVALID
     lib.kf_set_geometry.argtypes = [C.c_void_p, C.c_void_p, C.c_int32]
     assert lib.kf_set_geometry(None, None, 0) == abi.KF_E_INVALID
+
+
+def test_landau_sampler_matches_the_landau_distribution(built):
+    """The material simulation's straggling sampler (Chambers-Mallows-Stuck
+    for the index-1 totally skewed stable law) against the Landau CDF of
+    scipy.stats.landau (density (1/pi) int exp(-t ln t - x t) sin(pi t) dt,
+    the convention of the reference's CERNLIB-based sampler)."""
+    st = pytest.importorskip("scipy.stats")
+    x = G.landau_samples(200000, seed=11)
+    assert np.isfinite(x).all()
+    assert st.kstest(x, st.landau.cdf).pvalue > 1e-3
+    # quantiles, including the heavy right tail
+    for q in (0.01, 0.25, 0.5, 0.75, 0.99):
+        assert abs(np.quantile(x, q) - st.landau.ppf(q)) < 0.05 * max(1.0, abs(st.landau.ppf(q)))
+    # another stream gives other numbers
+    assert not np.array_equal(x[:100], G.landau_samples(100, seed=12))
