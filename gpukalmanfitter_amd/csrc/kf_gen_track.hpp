@@ -97,6 +97,27 @@ struct TrackRng {
     const double W = -log(uniform());
     return ((hp + V) * tan(V) - log(hp * W * cos(V) / (hp + V))) / hp;
   }
+  // Gamma(k, 1) variate, k > 0: the squeeze-rejection method of Marsaglia
+  // and Tsang (ACM TOMS 26 (2000) 363) for shape k + 1 when k < 1, times
+  // U^(1/k).  Each trial accepts with probability > 0.95; after 64 trials
+  // the last one is kept, so the loop always ends.
+  KFG_HD double gamma(double k) {
+    const double a = (k < 1.0) ? k + 1.0 : k;
+    const double d = a - 1.0 / 3.0, c = 1.0 / sqrt(9.0 * d);
+    double g = d;
+    for (int trial = 0; trial < 64; ++trial) {
+      const double x = normal();
+      double v = 1.0 + c * x;
+      if (v <= 0.0) continue;
+      v = v * v * v;
+      g = d * v;
+      const double u = uniform();
+      const double x2 = x * x;
+      if (u < 1.0 - 0.0331 * (x2 * x2) || log(u) < 0.5 * x2 + d * (1.0 - v + log(v))) break;
+    }
+    if (k < 1.0) g *= exp(log(uniform()) / k);
+    return g;
+  }
 };
 
 // ------------------------------------------------------------------ field ----
@@ -305,27 +326,42 @@ KFG_HD inline void simulateTrack(const kfg_config &cfg, const kf_plane *true_pla
       ++nh;
     }
     if (cfg.material && pl.thickness > 0.0) {
+      // the reference's simulation (ActsFatras MinimalSimulator): Highland
+      // scattering, then Bethe-Bloch (Landau) loss, then Bethe-Heitler loss,
+      // with the slab thickness scaled by the incidence
       const GenMaterial m{pl.thickness, pl.x0, pl.ar, pl.z, pl.molar_rho};
       const double pc = 1.0 / fabs(dot3(nrm, dir));
       double mpvLoss, sigmaE, theta0;
       materialEffects(m, cfg.mass, pp, pc, &mpvLoss, &sigmaE, &theta0);
-      double loss = mpvLoss + sigmaE * rs.landau();  // Landau(location, scale)
+      // scattering (Scattering.hpp, Highland.hpp): a deflection by theta ~
+      // N(0, sqrt(2) theta0) towards a uniformly oriented direction
+      // perpendicular to the track
+      {
+        const double psi = 6.283185307179586 * (rs.uniform() - 0.5);
+        const double th = 1.4142135623730951 * theta0 * rs.normal();
+        double a[3] = {0, 0, 1};
+        if (fabs(dir[2]) > 0.9) a[0] = 1, a[2] = 0;
+        double e1[3], e2[3];
+        cross3(dir, a, e1);
+        const double l1 = sqrt(dot3(e1, e1));
+        for (int i = 0; i < 3; ++i) e1[i] /= l1;
+        cross3(dir, e1, e2);
+        const double ct = cos(th), st = sin(th), cp = cos(psi), sn = sin(psi);
+        for (int i = 0; i < 3; ++i) dir[i] = ct * dir[i] + st * (cp * e1[i] + sn * e2[i]);
+        const double dn = sqrt(dot3(dir, dir));
+        for (int i = 0; i < 3; ++i) dir[i] /= dn;
+      }
+      // Bethe-Bloch (BetheBloch.hpp): Landau(most probable loss, width)
+      double loss = mpvLoss + sigmaE * rs.landau();
       if (loss < 0) loss = 0;  // the Landau's left tail, rare at these widths
-      const double E = sqrt(cfg.mass * cfg.mass + pp * pp) - loss;
+      double E = sqrt(cfg.mass * cfg.mass + pp * pp) - loss;
+      if (E <= cfg.mass * 1.0001) break;  // stopped
+      // Bethe-Heitler (BetheHeitler.hpp): E z with z = exp(-u),
+      // u ~ Gamma(t / ln 2) and t the path in radiation lengths
+      const double u = rs.gamma(pl.thickness * pc / pl.x0 / 0.6931471805599453);
+      E = E * exp(-u);
       if (E <= cfg.mass * 1.0001) break;  // stopped
       pp = sqrt(E * E - cfg.mass * cfg.mass);
-      // scattering: two orthogonal Gaussian deflections
-      double a[3] = {0, 0, 1};
-      if (fabs(dir[2]) > 0.9) a[0] = 1, a[2] = 0;
-      double e1[3], e2[3];
-      cross3(dir, a, e1);
-      const double l1 = sqrt(dot3(e1, e1));
-      for (int i = 0; i < 3; ++i) e1[i] /= l1;
-      cross3(dir, e1, e2);
-      const double t1 = theta0 * rs.normal(), t2 = theta0 * rs.normal();
-      for (int i = 0; i < 3; ++i) dir[i] += t1 * e1[i] + t2 * e2[i];
-      const double dn = sqrt(dot3(dir, dir));
-      for (int i = 0; i < 3; ++i) dir[i] /= dn;
     }
   }
   o.n_hits[it] = nh;

@@ -161,6 +161,14 @@ void kfg_philox(uint32_t *counter, const uint32_t *key) {
   ph.gen(counter);
 }
 
+void kfg_gamma(double k, uint64_t seed, int64_t n, double *out) {
+  const Philox ph{(uint32_t)seed, (uint32_t)(seed >> 32)};
+  for (int64_t i = 0; i < n; ++i) {
+    TrackRng r(ph, (uint32_t)i, 8u);
+    out[i] = r.gamma(k);
+  }
+}
+
 void kfg_landau(uint64_t seed, int64_t n, double *out) {
   const Philox ph{(uint32_t)seed, (uint32_t)(seed >> 32)};
   for (int64_t i = 0; i < n; ++i) {

@@ -64,6 +64,8 @@ void kfg_fill_grid(float *values, int32_t nx, int32_t ny, int32_t nz, const doub
 void kfg_philox(uint32_t *counter, const uint32_t *key);
 // n standard Landau variates of the generator's sampler (stream `seed`).
 void kfg_landau(uint64_t seed, int64_t n, double *out);
+// n Gamma(k, 1) variates of the generator's sampler (stream `seed`).
+void kfg_gamma(double k, uint64_t seed, int64_t n, double *out);
 
 // The same generator on the GPU (libkfgen_gpu.so, kf_generate_gpu.hip): one
 // thread per track, asynchronous on `stream` (a hipStream_t); every pointer is

@@ -135,6 +135,7 @@ def _lib():
         lib.kfg_fill_grid.argtypes = [P, C.c_int32, C.c_int32, C.c_int32, P, P, C.c_double, C.c_double]
         lib.kfg_philox.argtypes = [P, P]
         lib.kfg_landau.argtypes = [C.c_uint64, C.c_int64, P]
+        lib.kfg_gamma.argtypes = [C.c_double, C.c_uint64, C.c_int64, P]
         lib._kfg_typed = True
     return lib
 
@@ -243,6 +244,14 @@ def landau_samples(n, seed=1):
     straggling of the material simulation), for its distribution test."""
     out = np.empty(int(n), dtype=np.float64)
     _lib().kfg_landau(C.c_uint64(seed), int(n), _ptr(out))
+    return out
+
+
+def gamma_samples(k, n, seed=1):
+    """n Gamma(k, 1) variates of the generator's sampler (the Bethe-Heitler
+    loss of the material simulation), for its distribution test."""
+    out = np.empty(int(n), dtype=np.float64)
+    _lib().kfg_gamma(float(k), C.c_uint64(seed), int(n), _ptr(out))
     return out
 
 
