@@ -72,8 +72,9 @@ def test_config5_material_and_early_exit(built):
 
 def test_abi_library_exports_every_declared_symbol(built):
     header = open(os.path.join(ROOT, "include", "kf_abi.h")).read()
-    names = re.findall(r"^\s*(?:int|void|double|const char \*)\s*\*?\s*(kf_\w+)\s*\(", header, re.M)
-    assert len(names) >= 10
+    names = re.findall(r"^\s*(?:int|void|double|int64_t|int32_t|uint32_t|size_t|const char \*)"
+                       r"\s*\*?\s*(kf_\w+)\s*\(", header, re.M)
+    assert len(names) >= 20 and "kf_intersect_scratch_bytes" in names
     lib = C.CDLL(lib_path("libkfmi.so"))
     for n in names:
         assert hasattr(lib, n), n
