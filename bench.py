@@ -12,18 +12,18 @@ given by --precision (first one is the headline `value`).
 A fit is two launches on one stream: the forward filter (start, steps,
 gain-matrix update on every surface, per-surface store writes) and the
 smoother plus back-propagation to the perigee (store reads).  The roofline
-object is for the longer launch of the headline precision.  Both launches sit
-below the ridge point of the vector ALU (2.2-5.2 algorithmic flops per HBM
-byte against 10-14), so the bound is HBM: achieved = algorithmic bytes of the
-launch (inputs, outputs, smoother store, hand-over; ALG_BYTES_PER_TRACK) per
-second, peak 8 TB/s, traffic = the FETCH_SIZE + WRITE_SIZE bytes rocprofv3
-measured for the launch.  The compute side is reported beside it: flops are
-the algorithmic count of the reference formulation (structural zeros and
-identities removed) measured by the counting build of the CPU oracle, priced
-at F / (F32 / 157.3 + F64 / 78.6 TFLOP/s), the MI355X vector peaks weighted by
-the launch's float share.  Launch durations are the mean over the timed steps
-of HIP events the library records on the launch stream before, between and
-after the two launches.
+object is for the longer launch of the headline precision.  The path is bound
+by the vector ALU: the roofline time of a launch is T_roof = F32 / 157.3 +
+F64 / 78.6 TFLOP/s (the MI355X vector peaks) for its algorithmic flops, the
+flops of the reference formulation with structural zeros and identities
+removed, counted per track by the counting build of the CPU oracle and split
+into the float and double classes of the mixed mode; achieved = F / T, peak
+= F / T_roof, frac = T_roof / T.  Beside it: the launch's external I/O bytes
+(inputs it reads, results it writes) against 8 TB/s, the bytes of its
+working storage (smoother store, hand-over) and the HBM traffic rocprofv3
+measured (FETCH_SIZE + WRITE_SIZE).  Launch durations are the mean over the
+timed steps of HIP events the library records on the launch stream before,
+between and after the two launches.
 """
 import argparse
 import json
@@ -45,17 +45,19 @@ FLOPS32_PER_TRACK = 8109.0
 FLOPS32_BWD_PER_TRACK = 784.0
 FP64_PEAK_TFLOPS = 78.6
 FP32_PEAK_TFLOPS = 157.3
-# Algorithmic HBM bytes per config-2 track and launch (10 hits, 10 measured
-# states; tests/test_bench_constants.py re-derives them from the layouts):
+# External I/O bytes per config-2 track and launch (10 hits): what the caller
+# hands in and gets back, read or written by that launch.
 #   forward  = inputs 696 (start parameters 48 + covariance 168 + perigee 24
-#              + sequence 8 + hit offset 8 + 10 hits x 44) + chi2/ndf/n_measured
-#              16 + hand-over 264 (6 int32 + 30 double) + store writes
-#              10 x (12 x 8 + 49 x sizeof(S));
-#   backward = store reads 10 x (12 x 8 + 49 x sizeof(S)) + hand-over 264 +
-#              perigee, sequence and hit offsets 48 + outputs 220 (params 48,
+#              + sequence 8 + hit offset 8 + 10 hits x 44) + chi2/ndf/n_measured 16;
+#   backward = perigee, sequence and hit offsets 48 + outputs 220 (params 48,
 #              cov 168, status 4).
-ALG_BYTES_PER_TRACK = {("fp64", "forward"): 5856, ("fp64", "backward"): 5412,
-                       ("mixed", "forward"): 3896, ("mixed", "backward"): 3452}
+EXT_BYTES_PER_TRACK = {"forward": 712, "backward": 268}
+# The same plus the launch's working storage (tests/test_bench_constants.py
+# re-derives them from the layouts): the smoother store, 10 measured states x
+# (12 x 8 + 49 x sizeof(S)), written forward and read backward, and the
+# 264-byte forward-to-backward hand-over (6 int32 + 30 double).
+DESIGN_BYTES_PER_TRACK = {("fp64", "forward"): 5856, ("fp64", "backward"): 5412,
+                          ("mixed", "forward"): 3896, ("mixed", "backward"): 3452}
 HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch (fetched + written) of the two fit kernels on this
 # workload, from rocprofv3 FETCH_SIZE / WRITE_SIZE scaled by the copy-kernel
@@ -181,47 +183,57 @@ def param_errors(gpu_results, ores, n):
 
 
 def launch_figures(prec, phase, n, ms):
-    """Algorithmic bytes and flops of one launch of `phase` and what they make."""
+    """Flops, roofline time and bytes of one launch of `phase` (module docstring)."""
     mixed = prec == "mixed"
     f = FLOPS_PER_TRACK - FLOPS_BWD_PER_TRACK if phase == "forward" else FLOPS_BWD_PER_TRACK
     g = 0.0
     if mixed:
         g = FLOPS32_PER_TRACK - FLOPS32_BWD_PER_TRACK if phase == "forward" else FLOPS32_BWD_PER_TRACK
-    peak_flops = f / (g / FP32_PEAK_TFLOPS + (f - g) / FP64_PEAK_TFLOPS)  # TFLOP/s
-    b = ALG_BYTES_PER_TRACK[(prec, phase)]
+    t_roof = n * (g / (FP32_PEAK_TFLOPS * 1e12) + (f - g) / (FP64_PEAK_TFLOPS * 1e12))
     t = ms * 1e-3
+    ext = EXT_BYTES_PER_TRACK[phase]
+    design = DESIGN_BYTES_PER_TRACK[(prec, phase)]
     traffic = TRAFFIC_BYTES_PER_LAUNCH.get((prec, phase))
     if traffic is not None:
         traffic *= n / TRAFFIC_TRACKS
     return {"kernel": "kf_fit_kernel<%s, %s, true, false>" % ("float" if mixed else "double",
                                                             "true" if phase == "forward" else "false"),
-            "kernel_ms": ms, "bytes_per_track": b, "flops_per_track": f,
-            "fp32_flops_per_track": g, "intensity_flop_per_byte": f / b,
-            "hbm_gb_per_s": b * n / t / 1e9, "hbm_frac": b * n / t / 1e9 / HBM_PEAK_GBS,
-            "tflop_per_s": f * n / t / 1e12, "vector_peak_tflop_per_s": peak_flops,
-            "vector_frac": f * n / t / 1e12 / peak_flops, "traffic": traffic}
+            "kernel_ms": ms, "flops_per_track": f, "fp32_flops_per_track": g,
+            "t_roof_ms": t_roof * 1e3, "tflop_per_s": f * n / t / 1e12,
+            "vector_peak_tflop_per_s": f * n / t_roof / 1e12, "vector_frac": t_roof / t,
+            "hbm_ext": {"bytes_per_track": ext, "gb_per_s": ext * n / t / 1e9,
+                        "frac": ext * n / t / 1e9 / HBM_PEAK_GBS},
+            "hbm_design": {"bytes_per_track": design, "gb_per_s": design * n / t / 1e9,
+                           "frac": design * n / t / 1e9 / HBM_PEAK_GBS},
+            "traffic": traffic,
+            "traffic_over_ext": traffic / (ext * n) if traffic else None,
+            "traffic_over_design": traffic / (design * n) if traffic else None}
 
 
 def roofline(prec, n, phase_ms):
-    """Roofline of the dominant launch (see the module docstring)."""
+    """Vector-ALU roofline of the dominant launch (see the module docstring)."""
     ms = {"forward": phase_ms[0], "backward": phase_ms[1]}
     k = max(ms, key=ms.get)
     other = "backward" if k == "forward" else "forward"
     dom = launch_figures(prec, k, n, ms[k])
-    return {"bound": "hbm", "achieved": dom["hbm_gb_per_s"], "peak": HBM_PEAK_GBS,
-            "unit": "GB/s", "frac": dom["hbm_frac"], "traffic": dom["traffic"],
-            "kernel": dom["kernel"], "kernel_ms": ms[k],
-            "why_hbm": "arithmetic intensity %.2f flop/B is below the ridge point of the vector "
-                       "ALU (%.1f TFLOP/s / 8 TB/s = %.1f flop/B)" % (
-                           dom["intensity_flop_per_byte"], dom["vector_peak_tflop_per_s"],
-                           dom["vector_peak_tflop_per_s"] / 8.0),
+    return {"bound": "vector_alu", "achieved": dom["tflop_per_s"],
+            "peak": dom["vector_peak_tflop_per_s"], "unit": "TFLOP/s",
+            "frac": dom["vector_frac"], "traffic": dom["traffic"],
+            "kernel": dom["kernel"], "kernel_ms": ms[k], "t_roof_ms": dom["t_roof_ms"],
+            "peak_note": "F / (F32 / 157.3 + F64 / 78.6 TFLOP/s): the MI355X FP32 and FP64 "
+                         "vector peaks weighted by the launch's float and double flops",
+            "hbm": dict(dom["hbm_ext"], peak_gb_per_s=HBM_PEAK_GBS,
+                        note="external I/O of the launch (inputs read, results written) "
+                             "against 8 TB/s"),
             "dominant_launch": dom, "other_launch": launch_figures(prec, other, n, ms[other])}
 
 
-def adjoint_leg(steps, warmup):
+def adjoint_leg(steps, warmup, sample):
     """Gradient of the summed chi2 with respect to 60 alignment parameters and
     the field scale (kf_chi2_gradient) on the alignment workload: 262,144
-    tracks, 10 misaligned planes, field scale 1.003, no material effects."""
+    tracks, 10 misaligned planes, field scale 1.003, no material effects; then
+    the forward fit of the batch in both precisions.  Returns the figures, the
+    first `sample` tracks and both fits' results on them."""
     import numpy as np
     import torch
     from gpukalmanfitter_amd import configs
@@ -244,19 +256,39 @@ def adjoint_leg(steps, warmup):
     torch.cuda.synchronize()
     el = time.perf_counter() - t0
     _, _, _, flags, _ = kf.chi2_gradient(dev, opts)
+    # the forward fit of the same batch in both precisions (error vs the oracle)
+    fit_rates, res = {}, {}
+    for prec in ("mixed", "fp64"):
+        fopts = KalmanFitterOptions(precision=prec, **dict(configs.WORKLOAD_FIT_OPTIONS,
+                                                            multiple_scattering=False,
+                                                            energy_loss=False))
+        kf.fit_batch(dev, fopts)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            kf.fit_batch(dev, fopts)
+        torch.cuda.synchronize()
+        fit_rates[prec] = b.n_tracks * steps / (time.perf_counter() - t0)
+        r = dev.results_numpy()
+        res[prec] = {k: r[k][..., :sample].copy() for k in ("params", "status")}
     kf.close()
+    sub = b.subset(np.arange(min(sample, b.n_tracks)))
+    sub.field.scale = 1.003
     return {"workload": "config4-alignment: 262144 tracks, 10 misaligned x-normal planes, Bz=2 T "
                         "x 1.003, d(sum chi2)/d(60 alignment params + field scale)",
             "tracks_per_s": b.n_tracks * steps / el, "ms_per_step": el * 1e3 / steps,
             "kernel_ms": float(np.mean(kms)), "tracks_left_out": int((flags != 0).sum()),
-            "dtype": "f64"}
+            "dtype": "f64", "forward_fit_tracks_per_s": fit_rates}, sub, res
 
 
-def ragged_leg(steps, warmup, precision="mixed"):
+def ragged_leg(steps, warmup, sample, precision="mixed"):
     """The ragged batch: 2,000,000 tracks over 5-30 planes with 5 % holes in
     the 101x101x201 grid field, fitted with the schedule the library's
     autotuner picks on this batch (length sort, persistent work queue, graph
-    replay), beside the default one-thread-per-track schedule."""
+    replay), beside the default one-thread-per-track schedule, then in FP64
+    with the tuned schedule.  Returns the figures, the first `sample` tracks
+    and both fits' results on them."""
+    import numpy as np
     import torch
     from gpukalmanfitter_amd import configs
     from gpukalmanfitter_amd import generate as G
@@ -282,12 +314,21 @@ def ragged_leg(steps, warmup, precision="mixed"):
     default_s = timed()
     best, tune_ms = kf.autotune(dev, opts)
     tuned_s = timed()
+    r = dev.results_numpy()
+    res = {precision: {k: r[k][..., :sample].copy() for k in ("params", "status")}}
+    # the FP64 fit with the same schedule
+    opts = KalmanFitterOptions(precision="fp64", **configs.WORKLOAD_FIT_OPTIONS)
+    fp64_s = timed()
+    r = dev.results_numpy()
+    res["fp64"] = {k: r[k][..., :sample].copy() for k in ("params", "status")}
     kf.close()
     return {"workload": "config3-ragged-grid: 2000000 tracks, U{5..30} planes at 25 mm pitch, "
                         "5% holes, 101x101x201 FP32 grid field", "precision": precision,
             "tracks_per_s": b.n_tracks / tuned_s, "ms_per_step": tuned_s * 1e3,
             "schedule": best, "autotune_best_kernel_ms": tune_ms,
-            "default_schedule_tracks_per_s": b.n_tracks / default_s}
+            "default_schedule_tracks_per_s": b.n_tracks / default_s,
+            "fp64_same_schedule_tracks_per_s": b.n_tracks / fp64_s,
+            "mixed_over_fp64": fp64_s / tuned_s}, b.subset(np.arange(min(sample, b.n_tracks))), res
 
 
 def material_leg(steps, warmup, sample):
@@ -371,6 +412,34 @@ def material_leg(steps, warmup, sample):
           file=sys.stderr, flush=True)
     kf.close()
     return out, b.subset(np.arange(min(sample, b.n_tracks))), res
+
+
+def config1_leg(steps):
+    """The reference's CPU-runnable case: 10,000 tracks through the 10-plane
+    detector, fitted in both precisions; returns the figures, the batch and
+    the results (checked against the oracle in the CPU leg)."""
+    import torch
+    from gpukalmanfitter_amd import configs
+    from gpukalmanfitter_amd import generate as G
+    from gpukalmanfitter_amd.fitter import DeviceBatch, KalmanFitter, KalmanFitterOptions
+    b = G.generate(1)
+    kf = KalmanFitter(b.planes, b.field)
+    dev = DeviceBatch(b)
+    rates, res = {}, {}
+    for prec in ("mixed", "fp64"):
+        opts = KalmanFitterOptions(precision=prec, **configs.WORKLOAD_FIT_OPTIONS)
+        kf.fit_batch(dev, opts)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            kf.fit_batch(dev, opts)
+        torch.cuda.synchronize()
+        rates[prec] = b.n_tracks * steps / (time.perf_counter() - t0)
+        r = dev.results_numpy()
+        res[prec] = {k: r[k].copy() for k in ("params", "status")}
+    kf.close()
+    return {"workload": "config1-fp64-10k: 10000 tracks, 10 x-normal planes x=30..300 mm, "
+                        "Bz=2 T, no material", "tracks_per_s": rates}, b, res
 
 
 def end_to_end(kf, dev, opts, steps):
@@ -555,19 +624,23 @@ def main():
     print(f"[bench] end to end ({head}, H2D + fit + D2H): {e2e['tracks_per_s']:.4g} tracks/s, "
           f"{e2e['ms_per_step']:.2f} ms/step, {e2e['pcie_gb_per_s']:.1f} GB/s over PCIe",
           file=sys.stderr, flush=True)
+    others = {}
+    c1, c1b, c1r = config1_leg(max(args.steps // 2, 3))
+    others["config1"] = (c1b, c1r)
     ragged = None
     if not args.no_ragged:
-        ragged = ragged_leg(max(args.steps // 2, 3), 1)
+        ragged, rsub, rres = ragged_leg(max(args.steps // 2, 3), 1, 10000)
+        others["config3"] = (rsub, rres)
         print(f"[bench] ragged: {ragged['tracks_per_s']:.4g} tracks/s autotuned "
               f"({ragged['schedule']}), {ragged['default_schedule_tracks_per_s']:.4g} default",
               file=sys.stderr, flush=True)
     adj = None
     if not args.no_adjoint:
-        adj = adjoint_leg(max(args.steps // 2, 3), 1)
+        adj, asub, ares = adjoint_leg(max(args.steps // 2, 3), 1, 10000)
+        others["config4"] = (asub, ares)
         print(f"[bench] adjoint: {adj['tracks_per_s']:.4g} tracks/s, {adj['ms_per_step']:.3f} ms/step",
               file=sys.stderr, flush=True)
     material = None
-    others = {}
     if not args.no_material:
         material, msub, mres = material_leg(max(args.steps // 4, 2), 1, 10000)
         others["material"] = (msub, mres)
@@ -576,12 +649,25 @@ def main():
         inter, isample = intersect_leg(max(args.steps // 2, 3), 2)
     cpu = None
     errors = None
+    error_by_config = None
     if not args.no_cpu:
         cpu, errors, oerr = cpu_baseline(batch,
                                          abi.default_options(**configs.WORKLOAD_FIT_OPTIONS),
                                          args.cpu_threads, args.cpu_sample, results, others)
         if material is not None:
             material["error_vs_oracle"] = oerr["material"]
+        if ragged is not None:
+            ragged["error_vs_oracle"] = oerr["config3"]
+        if adj is not None:
+            adj["forward_fit_error_vs_oracle"] = oerr["config4"]
+        c1["error_vs_oracle"] = oerr["config1"]
+        summary = {"config2": errors}
+        summary.update({{"material": "config5"}.get(k, k): v for k, v in oerr.items()})
+        error_by_config = {
+            cfg_name: {prec: {"max_over_params": max(e["max"].values()),
+                              "status_equal": e["status_equal"]}
+                       for prec, e in errs.items() if isinstance(e, dict) and "max" in e}
+            for cfg_name, errs in sorted(summary.items())}
         if inter is not None:
             inter["cpu_baseline"], inter["oracle_bit_identical_on_sample"] = \
                 intersect_cpu_baseline(*isample, args.cpu_threads)
@@ -615,6 +701,8 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
         "error_vs_oracle": errors,
+        "error_vs_oracle_by_config": error_by_config,
+        "config1": c1,
         "ragged": ragged,
         "adjoint": adj,
         "material": material,

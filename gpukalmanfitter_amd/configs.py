@@ -56,6 +56,14 @@ _BASE = dict(
     true_field_scale=1.0,
 )
 
+# kfg_config.material: the reference's simulator, or the fit's own material
+# model (Gaussian projected scattering and Gaussian mean-Bethe loss, no
+# Bethe-Heitler) for pull tests of the material update, or that model's energy
+# loss alone (for fits without multiple scattering)
+MATERIAL_REFERENCE = 1
+MATERIAL_FIT_MODEL = 2
+MATERIAL_FIT_MODEL_ELOSS = 3
+
 GRID_FIELD = dict(
     kind="grid",
     n=(101, 101, 201),
@@ -85,7 +93,7 @@ CONFIGS = {
                      align_rot_sigma=1e-4), field=CONST_FIELD,
             field_scale_sigma=0.01, name="config4-alignment"),
     5: dict(gen=_cfg(n_tracks=4194304, seed=1234567895, n_planes=40, min_planes=8,
-                     max_planes=40, first_x=25.0, pitch=25.0, material=1,
+                     max_planes=40, first_x=25.0, pitch=25.0, material=MATERIAL_REFERENCE,
                      thickness=0.300, p_min=0.5, p_max=20.0),
             field=GRID_FIELD, name="config5-material"),
 }

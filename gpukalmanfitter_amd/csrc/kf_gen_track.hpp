@@ -233,12 +233,14 @@ KFG_HD inline bool propagateToPlane(const GenField &fld, double qop, double pos[
 // Landau distribution whose location is the most probable loss
 // (computeEnergyLossLandau, Interactions.ipp:220-240, RPP eq. 33.11) and
 // whose scale is the Gaussian-equivalent width of the Landau FWHM 4 eps
-// (computeEnergyLossLandauSigma, :298-314); Highland scattering width.
+// (computeEnergyLossLandauSigma, :298-314); Highland scattering width.  Also
+// the mean Bethe loss the fit subtracts (for the fit-model simulation).
 struct GenMaterial {
   double thickness, x0, ar, z, molarRho;
 };
 KFG_HD inline void materialEffects(const GenMaterial &m, double mass, double p, double pathCorr,
-                                   double *mpvLoss, double *sigmaE, double *theta0) {
+                                   double *mpvLoss, double *meanLoss, double *sigmaE,
+                                   double *theta0) {
   const double kMe = 0.5109989461e-3;
   const double kK = 0.307075e-3 * 100.0;
   const double x = m.thickness * pathCorr;
@@ -254,8 +256,10 @@ KFG_HD inline void materialEffects(const GenMaterial &m, double mass, double p, 
   const double wmax = 2 * kMe * bg * bg / (1.0 + 2 * gamma * mfrac + mfrac * mfrac);
   double dhalf = 0;
   if (bg >= 10.0) dhalf = log(bg) + log(28.816e-9 * sqrt(Ne) / I) - 0.5;
-  (void)u;
-  (void)wmax;
+  // the fit's mean loss: RPP eq. 34.5 over the slab, 2 eps [ln(u Wmax / I^2)/2
+  // - beta^2 - delta/2], with the reference's prefactor eps instead of 2 eps
+  // (Interactions.ipp:150-176)
+  *meanLoss = eps * (0.5 * log(u * wmax / (I * I)) - beta2 - dhalf);
   // the reference's most probable value takes the particle's mass in its
   // mass term (Interactions.ipp:235; RPP eq. 33.11 has the electron's)
   const double tMass = 2 * mass * bg * bg;
@@ -325,14 +329,35 @@ KFG_HD inline void simulateTrack(const kfg_config &cfg, const kf_plane *true_pla
       o.scratch_geo[it * maxp + nh] = pl.geo_id;
       ++nh;
     }
-    if (cfg.material && pl.thickness > 0.0) {
+    if ((cfg.material == KFG_MATERIAL_FIT_MODEL || cfg.material == KFG_MATERIAL_FIT_MODEL_ELOSS) &&
+        pl.thickness > 0.0) {
+      // the fit's own material model (a pull test of the material update):
+      // Gaussian projected scattering angles, theta by theta0 and phi by
+      // theta0 / sin(theta), and a Gaussian loss of the mean Bethe value and
+      // the Landau-FWHM width, both for the slab along the path
+      const GenMaterial m{pl.thickness, pl.x0, pl.ar, pl.z, pl.molar_rho};
+      const double pc = 1.0 / fabs(dot3(nrm, dir));
+      double mpvLoss, meanLoss, sigmaE, theta0;
+      materialEffects(m, cfg.mass, pp, pc, &mpvLoss, &meanLoss, &sigmaE, &theta0);
+      if (cfg.material == KFG_MATERIAL_FIT_MODEL) {
+        const double st = sqrt(dir[0] * dir[0] + dir[1] * dir[1]);
+        const double th = atan2(st, dir[2]) + theta0 * rs.normal();
+        const double ph = atan2(dir[1], dir[0]) + theta0 / st * rs.normal();
+        dir[0] = sin(th) * cos(ph);
+        dir[1] = sin(th) * sin(ph);
+        dir[2] = cos(th);
+      }
+      const double E = sqrt(cfg.mass * cfg.mass + pp * pp) - (meanLoss + sigmaE * rs.normal());
+      if (E <= cfg.mass * 1.0001) break;  // stopped
+      pp = sqrt(E * E - cfg.mass * cfg.mass);
+    } else if (cfg.material && pl.thickness > 0.0) {
       // the reference's simulation (ActsFatras MinimalSimulator): Highland
       // scattering, then Bethe-Bloch (Landau) loss, then Bethe-Heitler loss,
       // with the slab thickness scaled by the incidence
       const GenMaterial m{pl.thickness, pl.x0, pl.ar, pl.z, pl.molar_rho};
       const double pc = 1.0 / fabs(dot3(nrm, dir));
-      double mpvLoss, sigmaE, theta0;
-      materialEffects(m, cfg.mass, pp, pc, &mpvLoss, &sigmaE, &theta0);
+      double mpvLoss, meanLoss, sigmaE, theta0;
+      materialEffects(m, cfg.mass, pp, pc, &mpvLoss, &meanLoss, &sigmaE, &theta0);
       // scattering (Scattering.hpp, Highland.hpp): a deflection by theta ~
       // N(0, sqrt(2) theta0) towards a uniformly oriented direction
       // perpendicular to the track

@@ -11,6 +11,17 @@
 extern "C" {
 #endif
 
+// Material simulation of kfg_config.material.  REFERENCE: the reference's
+// simulator (Highland space-angle deflection, Landau Bethe-Bloch loss at the
+// most probable value, Bethe-Heitler loss).  FIT_MODEL: exactly what the fit's
+// material update assumes (Gaussian projected angles of width theta0 and
+// theta0 / sin(theta), Gaussian loss of the mean Bethe value and the Landau
+// FWHM width, nothing else), for pull tests of the material update;
+// FIT_MODEL_ELOSS: its energy loss alone (for fits without multiple scattering).
+#define KFG_MATERIAL_REFERENCE 1
+#define KFG_MATERIAL_FIT_MODEL 2
+#define KFG_MATERIAL_FIT_MODEL_ELOSS 3
+
 // Generator configuration.  Planes: plane i of the detector sits at
 // x = first_x + i * pitch with normal +x (local u = y, v = z).
 typedef struct kfg_config {
@@ -19,7 +30,8 @@ typedef struct kfg_config {
   int32_t n_planes;      // planes in the detector
   int32_t min_planes;    // per-track sequence length ~ U{min_planes..max_planes}
   int32_t max_planes;
-  int32_t material;      // 1: planes carry silicon of `thickness` (simulated + in the table)
+  int32_t material;      // 0: none; planes carry silicon of `thickness` (in the table and
+                         // simulated) with KFG_MATERIAL_REFERENCE or KFG_MATERIAL_FIT_MODEL
   double first_x, pitch, half_u, half_v;
   double thickness;      // mm
   double hole_prob;      // probability to drop a hit

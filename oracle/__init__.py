@@ -54,6 +54,9 @@ def _lib(counting=False):
     lib.kfo_material.argtypes = [C.c_int32] + [C.c_double] * 9
     lib.kfo_material.restype = C.c_double
     lib.kfo_field_get.argtypes = [C.POINTER(kfo_field), P, P]
+    lib.kfo_material_interact.argtypes = [C.POINTER(abi.kf_plane), P, P, C.c_double, C.c_int32,
+                                          C.c_double, P, C.c_int32, C.c_int32, C.c_int32,
+                                          C.c_int32, P, P]
     lib.kfo_intersect.argtypes = [P, C.c_int64, P, P, C.c_int64, C.c_int32, P, C.c_int64,
                                   C.c_int32]
     lib.kfo_intersect.restype = C.c_int64
@@ -203,6 +206,21 @@ def material(which, slab, mass, qop, q):
     """which: 'bethe' | 'highland' | 'landau_qop'; slab = (thickness, x0, l0, ar, z, molar_rho)."""
     w = {"bethe": 0, "highland": 1, "landau_qop": 2}[which]
     return _lib(False).kfo_material(w, *map(float, slab), float(mass), float(qop), float(q))
+
+
+def material_interact(plane, pos, direction, p, q, mass, cov, nav=1, stage=0,
+                      multiple_scattering=True, energy_loss=True):
+    """One material-interactor call on ``plane`` (an abi.kf_plane): returns the
+    momentum and the 6x6 bound covariance after it."""
+    c = np.ascontiguousarray(cov, dtype=np.float64).reshape(36)
+    out = np.zeros(36)
+    p_out = C.c_double()
+    _lib(False).kfo_material_interact(
+        C.byref(plane), _p(np.ascontiguousarray(pos, dtype=np.float64)),
+        _p(np.ascontiguousarray(direction, dtype=np.float64)), float(p), int(q), float(mass),
+        _p(c), int(nav), int(stage), int(bool(multiple_scattering)), int(bool(energy_loss)),
+        C.byref(p_out), _p(out))
+    return p_out.value, out.reshape(6, 6)
 
 
 def field_get(field, pos):

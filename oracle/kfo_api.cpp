@@ -423,6 +423,42 @@ double kfo_material(int32_t which, double thickness, double x0, double l0, doubl
   return kval(landauSigmaQOverP(s, mass, qop, q));
 }
 
+// One Actor::materialInteractor call (KalmanFitter.hpp:405-449) on the plane
+// `plane` for a stepper state at pos/dir with momentum p, charge q and bound
+// covariance cov[36] (row-major), navigation direction nav (+1 / -1).  stage:
+// 0 = the plane is an intermediate surface, 1 = it is the navigation's start
+// surface, -1 = its target surface (the split-factor cases of
+// PointwiseMaterialInteraction.hpp:87-108).  Writes the momentum and the bound
+// covariance after the call.
+void kfo_material_interact(const kf_plane *plane, const double pos[3], const double dir[3],
+                           double p, int32_t q, double mass, const double cov[36], int32_t nav,
+                           int32_t stage, int32_t multiple_scattering, int32_t energy_loss,
+                           double *p_out, double cov_out[36]) {
+  Surface surf = makePlane(*plane);
+  KalmanFitter kf(nullptr);
+  PropState st;
+  st.options.mass = mass;
+  StepperState &s = st.stepping;
+  s.pos = v3(pos[0], pos[1], pos[2]);
+  s.dir = v3(dir[0], dir[1], dir[2]);
+  s.p = p;
+  s.q = q;
+  s.navDir = nav;
+  s.covTransport = true;
+  for (int a = 0; a < 6; ++a)
+    for (int b = 0; b < 6; ++b) s.cov(a, b) = cov[6 * a + b];
+  st.navigation.currentSurface = &surf;
+  st.navigation.startSurface = (stage == 1) ? &surf : nullptr;
+  st.navigation.targetSurface = (stage == -1) ? &surf : nullptr;
+  FitOptions fo;
+  fo.multipleScattering = multiple_scattering != 0;
+  fo.energyLoss = energy_loss != 0;
+  kf.materialInteractor(&surf, st, fo);
+  *p_out = kval(s.p);
+  for (int a = 0; a < 6; ++a)
+    for (int b = 0; b < 6; ++b) cov_out[6 * a + b] = kval(s.cov(a, b));
+}
+
 // Field lookup (for the grid trilinear tests).
 void kfo_field_get(const kfo_field *field, const double pos[3], double b[3]) {
   Field fl = makeField(field);

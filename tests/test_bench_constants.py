@@ -27,8 +27,9 @@ def test_flops_per_track_constants(built):
         assert abs(measured.mean() - c) / c < 0.01, (name, measured.mean())
 
 
-def test_algorithmic_bytes_per_track_constants():
-    """bench.ALG_BYTES_PER_TRACK from the store layout of the kernels
+def test_bytes_per_track_constants():
+    """bench.EXT_BYTES_PER_TRACK and DESIGN_BYTES_PER_TRACK from the ABI arrays
+    and the store layout of the kernels
     (include/kf_abi.h, gpukalmanfitter_amd/csrc/kf_kernels.hpp) and the
     config-2 track shape (10 hits, 10 measured states)."""
     src = open(os.path.join(ROOT, "bench.py")).read()
@@ -38,13 +39,17 @@ def test_algorithmic_bytes_per_track_constants():
     # 16-byte group; padding is not algorithmic traffic)
     ss = int(re.search(r"#define KF_SS_USED (\d+)", hdr).group(1))
     import ast
-    table = ast.literal_eval(re.search(r"ALG_BYTES_PER_TRACK = (\{.*?\})", src, re.S).group(1))
+    table = ast.literal_eval(re.search(r"DESIGN_BYTES_PER_TRACK = (\{.*?\})", src, re.S).group(1))
+    ext = ast.literal_eval(re.search(r"EXT_BYTES_PER_TRACK = (\{.*?\})", src, re.S).group(1))
     b = G.generate(2, n_tracks=1000)
     assert (np.diff(b.hit_offset) == 10).all() and (b.seq_count == 10).all()
     hits = 10 * (4 + 2 * 8 + 3 * 8)
     inputs = 6 * 8 + 21 * 8 + 3 * 8 + 2 * 4 + 8 + hits
     carry = 6 * 4 + 30 * 8
+    outputs = 6 * 8 + 21 * 8 + 4
+    assert ext["forward"] == inputs + 16
+    assert ext["backward"] == 48 + outputs
     for prec, ssize in (("fp64", 8), ("mixed", 4)):
         store = 10 * (sd * 8 + ss * ssize)
         assert table[(prec, "forward")] == inputs + 16 + carry + store
-        assert table[(prec, "backward")] == store + carry + 48 + (6 * 8 + 21 * 8 + 4)
+        assert table[(prec, "backward")] == store + carry + 48 + outputs
