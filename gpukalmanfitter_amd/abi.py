@@ -5,7 +5,7 @@ calling it is done by :mod:`gpukalmanfitter_amd.fitter`.
 """
 import ctypes as C
 
-KF_ABI_VERSION = 2
+KF_ABI_VERSION = 3
 
 KF_OK = 0
 KF_E_INVALID = -1

@@ -51,6 +51,7 @@ struct kf_ctx {
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
   std::vector<char> gkey;
+  int64_t captures = 0;
 };
 
 namespace {
@@ -285,7 +286,9 @@ int setupArgs(kf_ctx *ctx, const kf_tracks *tr, const kf_options *opt, const cha
   a.planes = ctx->planes;
   a.n_planes = ctx->n_planes;
   a.unique_geo = ctx->unique_geo ? 1 : 0;
-  a.field = ctx->field;
+  // byte copy: the padding of the field description is ctx's (zeroed), so
+  // equal arguments give equal graph-replay keys
+  std::memcpy(&a.field, &ctx->field, sizeof a.field);
   a.smoothing = opt->smoothing;
   a.ms = opt->multiple_scattering;
   a.eloss = opt->energy_loss;
@@ -323,31 +326,50 @@ int64_t sliceTracks(const kf_ctx *ctx, int64_t n, size_t bytes_per_track) {
   return m >= n ? n : m;
 }
 
-// Per-state stores (layout in kf_kernels.hpp) and the forward -> backward
-// hand-over for slices of up to `slice` positions, carved from the
-// context's buffers.
+// Byte sizes of the working storage for slices of up to `slice` positions:
+// the per-state stores (layout in kf_kernels.hpp), each 256-byte aligned, and
+// the forward -> backward hand-over.
+struct StoreLayout {
+  size_t s_bytes, d_bytes, f_bytes, c_bytes, carry_bytes;
+  size_t store() const { return s_bytes + d_bytes + f_bytes + c_bytes; }
+};
+StoreLayout storeLayout(int precision, bool track_states, bool checkpoints, int64_t slice,
+                        int32_t max_states) {
+  const size_t slots = ((size_t)slice + 63) / 64 * 64 * (size_t)max_states;
+  const size_t s_size = (precision == KF_PRECISION_FP64) ? sizeof(double) : sizeof(float);
+  StoreLayout L;
+  L.s_bytes = (slots * KF_SS_N * s_size + 255) / 256 * 256;
+  L.d_bytes = slots * KF_SD_N * sizeof(double);
+  L.f_bytes = track_states ? slots * KF_TS_STRIDE * sizeof(double) : 0;
+  L.c_bytes = checkpoints ? slots * KF_CK_N * sizeof(double) : 0;
+  L.carry_bytes = ((size_t)slice * kCarryBytes + 255) / 256 * 256;
+  return L;
+}
+
+// Points the kernel arguments at stores carved from `store` and the
+// hand-over buffer `carry`.
+void carveStores(const StoreLayout &L, void *store, void *carry, bool track_states,
+                 bool checkpoints, kfd::FitArgs &a) {
+  char *base = (char *)store;
+  a.tss = base;
+  a.tsd = (double *)(base + L.s_bytes);
+  a.tsf = track_states ? (double *)(base + L.s_bytes + L.d_bytes) : nullptr;
+  a.tsc = checkpoints ? (double *)(base + L.s_bytes + L.d_bytes + L.f_bytes) : nullptr;
+  // [field][count] of each slice (sliceCarry)
+  a.carry_d = (double *)carry;
+  a.carry_i = nullptr;
+}
+
+// The working storage for slices of up to `slice` positions, from the
+// context's buffers (grown as needed).
 int setupStores(kf_ctx *ctx, int precision, bool track_states, bool checkpoints, int64_t slice,
                 hipStream_t s, kfd::FitArgs &a) {
-  const size_t slots = ((size_t)slice + 63) / 64 * 64 * (size_t)a.max_states;
-  const size_t s_size = (precision == KF_PRECISION_FP64) ? sizeof(double) : sizeof(float);
-  const size_t d_bytes = slots * KF_SD_N * sizeof(double);
-  const size_t s_bytes = (slots * KF_SS_N * s_size + 255) / 256 * 256;
-  const size_t f_bytes = track_states ? slots * KF_TS_STRIDE * sizeof(double) : 0;
-  const size_t c_bytes = checkpoints ? slots * KF_CK_N * sizeof(double) : 0;
-  int rc = grow(ctx, &ctx->store, &ctx->store_bytes, d_bytes + s_bytes + f_bytes + c_bytes, s,
-                "state store");
+  const StoreLayout L = storeLayout(precision, track_states, checkpoints, slice, a.max_states);
+  int rc = grow(ctx, &ctx->store, &ctx->store_bytes, L.store(), s, "state store");
   if (rc != KF_OK) return rc;
-  char *base = (char *)ctx->store;
-  a.tss = base;
-  a.tsd = (double *)(base + s_bytes);
-  a.tsf = track_states ? (double *)(base + s_bytes + d_bytes) : nullptr;
-  a.tsc = checkpoints ? (double *)(base + s_bytes + d_bytes + f_bytes) : nullptr;
-  rc = grow(ctx, &ctx->carry, &ctx->carry_bytes, (size_t)slice * kCarryBytes, s,
-            "hand-over buffer");
+  rc = grow(ctx, &ctx->carry, &ctx->carry_bytes, L.carry_bytes, s, "hand-over buffer");
   if (rc != KF_OK) return rc;
-  // [field][count] of each slice (sliceCarry)
-  a.carry_d = (double *)ctx->carry;
-  a.carry_i = nullptr;
+  carveStores(L, ctx->store, ctx->carry, track_states, checkpoints, a);
   return KF_OK;
 }
 
@@ -374,22 +396,19 @@ hipError_t launchSlices(kfd::FitArgs a, const KfdLaunch &L, int precision, doubl
   return e;
 }
 
-}  // namespace
-
-extern "C" {
-
-int kf_fit(kf_ctx *ctx, const kf_tracks *tr, const kf_options *opt, const kf_results *res,
-           void *stream) {
-  if (!ctx) return fail(nullptr, KF_E_INVALID, "kf_fit: ctx is NULL");
-  if (!tr || !opt || !res) return fail(ctx, KF_E_INVALID, "kf_fit: NULL tracks/options/results");
-  kfd::FitArgs a;
-  int rc = setupArgs(ctx, tr, opt, "kf_fit", a);
+// Checks the fit's arguments and fills the kernel arguments (inputs, options,
+// outputs); makes ctx's device current.
+int prepareFit(kf_ctx *ctx, const kf_tracks *tr, const kf_options *opt, const kf_results *res,
+               const char *who, kfd::FitArgs &a) {
+  if (!ctx) return fail(nullptr, KF_E_INVALID, "%s: ctx is NULL", who);
+  if (!tr || !opt || !res) return fail(ctx, KF_E_INVALID, "%s: NULL tracks/options/results", who);
+  int rc = setupArgs(ctx, tr, opt, who, a);
   if (rc != KF_OK) return rc;
   if (tr->n_tracks == 0) return KF_OK;
   if (!res->params || !res->cov || !res->status)
-    return fail(ctx, KF_E_INVALID, "kf_fit: results need params, cov and status");
+    return fail(ctx, KF_E_INVALID, "%s: results need params, cov and status", who);
   if (res->track_states && !res->n_measured)
-    return fail(ctx, KF_E_INVALID, "kf_fit: track_states needs results.n_measured");
+    return fail(ctx, KF_E_INVALID, "%s: track_states needs results.n_measured", who);
   a.out_params = res->params;
   a.out_cov = res->cov;
   a.out_chi2 = res->chi2;
@@ -397,6 +416,18 @@ int kf_fit(kf_ctx *ctx, const kf_tracks *tr, const kf_options *opt, const kf_res
   a.out_nmeas = res->n_measured;
   a.out_status = res->status;
   KF_HIP(ctx, hipSetDevice(ctx->device));
+  return KF_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int kf_fit(kf_ctx *ctx, const kf_tracks *tr, const kf_options *opt, const kf_results *res,
+           void *stream) {
+  kfd::FitArgs a;
+  int rc = prepareFit(ctx, tr, opt, res, "kf_fit", a);
+  if (rc != KF_OK || tr->n_tracks == 0) return rc;
   hipStream_t s = (hipStream_t)stream;
   const bool ts = res->track_states != nullptr;
   const int64_t slice = sliceTracks(ctx, a.n, storeBytesPerTrack(a, opt->precision, ts, false));
@@ -454,6 +485,7 @@ int kf_fit(kf_ctx *ctx, const kf_tracks *tr, const kf_options *opt, const kf_res
       ctx->graph = g;
       KF_HIP(ctx, hipGraphInstantiate(&ctx->gexec, g, nullptr, nullptr, 0));
       ctx->gkey = key;
+      ++ctx->captures;
     }
     KF_HIP(ctx, hipEventRecord(ctx->ev0, s));
     e = hipGraphLaunch(ctx->gexec, s);
@@ -649,21 +681,23 @@ int64_t kf_intersect_scratch_bytes(int64_t n_rays, int64_t n_surfaces) {
 
 int kf_intersect(const kf_polygon_plane *surfaces, int64_t n_surfaces, const double *ray_pos,
                  const double *ray_dir, int64_t n_rays, int32_t bcheck, kf_intersection *hits,
-                 int64_t capacity, int64_t *n_hits, void *scratch, void *stream) {
+                 int64_t capacity, int64_t *n_hits, void *work, void *stream) {
   if (n_surfaces < 0 || n_rays < 0 || capacity < 0)
     return fail(nullptr, KF_E_INVALID, "kf_intersect: negative size");
   if (n_surfaces > INT32_MAX || n_rays > INT32_MAX)
     return fail(nullptr, KF_E_INVALID, "kf_intersect: more than 2^31 - 1 rays or surfaces");
-  if (!n_hits || !scratch || (n_rays > 0 && (!ray_pos || !ray_dir)) ||
+  if (!n_hits || !work || (n_rays > 0 && (!ray_pos || !ray_dir)) ||
       (n_surfaces > 0 && !surfaces) || (capacity > 0 && !hits))
     return fail(nullptr, KF_E_INVALID, "kf_intersect: NULL array");
   const hipError_t e = kfd_launch_intersect(surfaces, n_surfaces, ray_pos, ray_dir, n_rays,
-                                            bcheck, hits, capacity, n_hits, scratch,
+                                            bcheck, hits, capacity, n_hits, work,
                                             (hipStream_t)stream);
   if (e != hipSuccess)
     return fail(nullptr, KF_E_HIP, "kf_intersect: launch failed: %s", hipGetErrorString(e));
   return KF_OK;
 }
+
+int64_t kf_graph_captures(const kf_ctx *ctx) { return ctx ? ctx->captures : 0; }
 
 double kf_last_kernel_ms(const kf_ctx *ctx) {
   if (!ctx || !ctx->timed) return -1.0;

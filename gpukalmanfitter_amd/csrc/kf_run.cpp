@@ -348,11 +348,13 @@ int main(int argc, char **argv) {
   };
   std::vector<Chunk> chunks(nStreams);
   std::vector<std::unique_ptr<kfmi::KalmanFitter>> fitters;
-  const int64_t per = (n + nStreams - 1) / nStreams;
+  // n / nStreams tracks per stream, one more for the first n % nStreams
+  // streams: every chunk holds at least one track (nStreams <= n)
+  const int64_t per = n / nStreams, extra = n % nStreams;
   for (int s = 0; s < nStreams; ++s) {
     Chunk &c = chunks[s];
-    c.first = s * per;
-    c.count = std::min(per, n - c.first);
+    c.first = s * per + std::min<int64_t>(s, extra);
+    c.count = per + (s < extra ? 1 : 0);
     auto slice = [&](const std::vector<double> &v, int comps) {
       std::vector<double> out((size_t)comps * c.count);
       for (int k = 0; k < comps; ++k)
@@ -525,14 +527,37 @@ int main(int argc, char **argv) {
     return 1;
   }
   if (withCopies) {
-    // the copied-out results must equal the device results
+    // every copied-out result array must equal the device's, element for
+    // element (per chunk: [6][count] params, [21][count] cov, then chi2, ndf,
+    // n_measured and status of its tracks)
     int64_t diff = 0;
+    std::vector<double> parDev(6 * n), covDev(21 * n), chiDev(n);
+    std::vector<int32_t> ndfDev(n), nmDev(n);
     std::vector<uint32_t> stDev(n);
+    toHost(parDev, dpar);
+    toHost(covDev, dcov);
+    toHost(chiDev, dchi);
+    toHost(ndfDev, dndf);
+    toHost(nmDev, dnm);
     toHost(stDev, dst);
-    for (int s = 0; s < nStreams; ++s)
-      for (int64_t j = 0; j < chunks[s].count; ++j)
-        diff += (hchunks[s].st[j] != stDev[chunks[s].first + j]) ? 1 : 0;
-    std::printf("INFO: copied-out statuses differing from the device's: %lld\n", (long long)diff);
+    auto differ = [](const void *a, const void *b, size_t bytes) {
+      return std::memcmp(a, b, bytes) != 0;
+    };
+    for (int s = 0; s < nStreams; ++s) {
+      const Chunk &c = chunks[s];
+      const HostChunk &h = hchunks[s];
+      for (int64_t j = 0; j < 6 * c.count; ++j)
+        diff += differ(&h.par[j], &parDev[6 * c.first + j], sizeof(double));
+      for (int64_t j = 0; j < 21 * c.count; ++j)
+        diff += differ(&h.cov[j], &covDev[21 * c.first + j], sizeof(double));
+      for (int64_t j = 0; j < c.count; ++j) {
+        diff += differ(&h.chi[j], &chiDev[c.first + j], sizeof(double));
+        diff += (h.ndf[j] != ndfDev[c.first + j]) + (h.nm[j] != nmDev[c.first + j]) +
+                (h.st[j] != stDev[c.first + j]);
+      }
+    }
+    std::printf("INFO: copied-out result elements differing from the device's: %lld\n",
+                (long long)diff);
   }
 
   // ---- results ----

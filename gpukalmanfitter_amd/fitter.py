@@ -54,6 +54,8 @@ def _lib():
         lib.kf_last_kernel_ms.restype = C.c_double
         lib.kf_last_phase_ms.argtypes = [P, P, P]
         lib.kf_last_phase_ms.restype = C.c_int
+        lib.kf_graph_captures.argtypes = [P]
+        lib.kf_graph_captures.restype = C.c_int64
         lib.kf_chi2_gradient.argtypes = [P, C.POINTER(abi.kf_tracks), C.POINTER(abi.kf_options),
                                          P, P, P, P, P, P]
         lib.kf_chi2_gradient.restype = C.c_int
@@ -399,6 +401,11 @@ class KalmanFitter:
         f, b = C.c_double(), C.c_double()
         self._check(self._lib.kf_last_phase_ms(self._ctx, C.byref(f), C.byref(b)))
         return f.value, b.value
+
+    def graph_captures(self):
+        """Launch sequences captured as hipGraphs so far (a replay of unchanged
+        arguments captures nothing)."""
+        return int(self._lib.kf_graph_captures(self._ctx))
 
     # ------------------------------------------------------- per-track API --
     def fit(self, sourcelinks, start_parameters, kf_options, perigee,

@@ -58,7 +58,7 @@
 extern "C" {
 #endif
 
-#define KF_ABI_VERSION 2
+#define KF_ABI_VERSION 3
 
 /* return codes */
 #define KF_OK 0
@@ -118,8 +118,10 @@ typedef struct kf_options {
   int32_t abs_pdg;             /* default 13 */
   int32_t max_steps;           /* default 1000 */
   int32_t max_rk_trials;       /* default 10000 */
-  int32_t max_states;          /* capacity of the per-track state store; 0 = use the
-                                  longest surface sequence of the batch */
+  int32_t max_states;          /* capacity of the per-track state store, at least the
+                                  longest surface sequence of the batch; 0 = the size
+                                  of the plane table (which holds every sequence);
+                                  the store and the memory slices are sized by it */
   double mass;                 /* default 105.6583755 MeV = 0.1056583755 */
   double tolerance;            /* RK error tolerance, default 1e-4 */
   double step_size_cutoff;     /* default 0 */
@@ -273,6 +275,12 @@ double kf_last_kernel_ms(const kf_ctx *ctx);
  * counterpart (the reference times the whole batch from the host,
  * Run/GPU/KalmanFitterGPUTest.cu:382-513). */
 int kf_last_phase_ms(const kf_ctx *ctx, double *forward_ms, double *backward_ms);
+
+/* Number of launch sequences ctx has captured as a hipGraph (kf_schedule.graph);
+ * a kf_fit whose arguments equal the previous call's replays the captured
+ * graph and captures nothing.  No reference counterpart (the reference has no
+ * graph replay); for callers that check their calls replay. */
+int64_t kf_graph_captures(const kf_ctx *ctx);
 
 /* ---- Batched ray x bounded-plane intersection ------------------------------
  * The reference's intersection test (Run/GPU/IntersectGPUTest.cu:22-48,

@@ -65,8 +65,8 @@ def test_driver_fit_matches_oracle_and_per_track_facade(built, tmp_path):
 def test_driver_with_copies_gives_the_same_fit(built, tmp_path):
     """kf_run --with-copies (each stream copies its chunk in, fits it and
     copies its results out inside the timed loop) fits the same tracks to the
-    same results as the resident-input run, and its copied-out statuses equal
-    the device's."""
+    same results as the resident-input run, and every copied-out result array
+    (params, cov, chi2, ndf, n_measured, status) equals the device's."""
     n = 20000
     a, b = tmp_path / "a", tmp_path / "b"
     a.mkdir()
@@ -75,6 +75,6 @@ def test_driver_with_copies_gives_the_same_fit(built, tmp_path):
     pb = run(["-c", "3", "-t", str(n), "-o", "1", "-n", "2", "-e", "3", "-p", "mixed",
               "--with-copies"], cwd=b)
     assert pa.returncode == 0 and pb.returncode == 0, pa.stderr + pb.stderr
-    assert "copied-out statuses differing from the device's: 0" in pb.stdout, pb.stdout
+    assert "copied-out result elements differing from the device's: 0" in pb.stdout, pb.stdout
     name = f"params-fitted_config3_{n}.csv"
     assert (a / name).read_bytes() == (b / name).read_bytes()

@@ -390,9 +390,37 @@ def test_schedules_give_identical_results(torch_cuda, sched):
         got = dev.results_numpy()
         for k in ("params", "cov", "chi2", "status", "n_measured", "track_states"):
             assert np.array_equal(got[k], ref[k]), k
+    # identical calls replay the one capture
+    assert kf.graph_captures() == (1 if sched.get("graph") else 0)
     kf.close()
     o, _, _, _ = oracle.fit(b, c_options(**W))
     check_parity(ref, o)
+
+
+def test_graph_recaptures_only_when_arguments_change(torch_cuda):
+    """The replay key holds every argument of the captured launches: equal
+    calls replay, a changed option or output array captures again."""
+    from gpukalmanfitter_amd.fitter import DeviceBatch, KalmanFitter, KalmanFitterOptions
+    b = G.generate(3, n_tracks=3000)
+    kf = KalmanFitter(b.planes, b.field)
+    kf.set_schedule(graph=1)
+    dev = DeviceBatch(b)
+    opts = KalmanFitterOptions(**W)
+    for _ in range(4):
+        kf.fit_batch(dev, opts)
+    assert kf.graph_captures() == 1
+    ref = dev.results_numpy()
+    kf.fit_batch(dev, KalmanFitterOptions(precision="mixed", **W))
+    assert kf.graph_captures() == 2
+    kf.fit_batch(dev, opts)
+    kf.fit_batch(dev, opts)
+    assert kf.graph_captures() == 3
+    dev2 = DeviceBatch(b)
+    kf.fit_batch(dev2, opts)
+    assert kf.graph_captures() == 4
+    for k in ("params", "cov", "status"):
+        assert np.array_equal(dev2.results_numpy()[k], ref[k]), k
+    kf.close()
 
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
