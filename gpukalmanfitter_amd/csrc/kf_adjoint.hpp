@@ -520,8 +520,9 @@ struct PlaneFwd {
     l31 = sinP * cosT;
     l32 = -sinT;
     dp = nr[0] * d[0] + nr[1] * d[1] + nr[2] * d[2];
+    const double idp = 1.0 / dp;
 #pragma unroll
-    for (int k = 0; k < 3; ++k) nv[k] = nr[k] / dp;
+    for (int k = 0; k < 3; ++k) nv[k] = nr[k] * idp;
 #pragma unroll
     for (int c = 0; c < 5; ++c) {
       sv[c] = nv[0] * A[0][c] + nv[1] * A[1][c] + nv[2] * A[2][c];
@@ -570,9 +571,10 @@ struct PlaneFwd {
     V11 = v11;
     const double S00 = Cp[0][0] + V00, S01 = Cp[0][1] + V01, S11 = Cp[1][1] + V11;
     const double det = S00 * S11 - S01 * S01;
-    Si00 = S11 / det;
-    Si01 = -S01 / det;
-    Si11 = S00 / det;
+    const double idet = 1.0 / det;
+    Si00 = S11 * idet;
+    Si01 = -S01 * idet;
+    Si11 = S00 * idet;
     r[0] = m0 - xp[0];
     r[1] = m1 - xp[1];
     u[0] = Si00 * r[0] + Si01 * r[1];

@@ -237,6 +237,7 @@ int kf_set_field_grid(kf_ctx *ctx, const float *b, int32_t nx, int32_t ny, int32
     ctx->field.lo[d] = lo[d];
     ctx->field.hi[d] = hi[d];
     ctx->field.w[d] = (hi[d] - lo[d]) / (n[d] - 1);
+    ctx->field.iw[d] = 1.0 / ctx->field.w[d];
   }
   ctx->field.scale = scale;
   ctx->field_set = true;

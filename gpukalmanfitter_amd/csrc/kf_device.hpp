@@ -58,6 +58,7 @@ struct DevField {
   const float *values;
   int n[3];
   double lo[3], hi[3], w[3];
+  double iw[3];  // 1 / w: cell index and fraction by multiplication
 };
 
 // Field lookup in double.  Grid: trilinear, last axis first (the reference's
@@ -83,13 +84,12 @@ KFD_INLINE void fieldAt(const DevField &f, double x, double y, double z, double 
     double v = p[d];
     if (v < f.lo[d]) v = f.lo[d];
     if (f.hi[d] < v) v = f.hi[d];
-    int i = (int)floor((v - f.lo[d]) / f.w[d]);
+    int i = (int)floor((v - f.lo[d]) * f.iw[d]);
     i = i < 0 ? 0 : i;
     i = i > f.n[d] - 2 ? f.n[d] - 2 : i;
     idx[d] = i;
     const double ll = f.lo[d] + (double)i * f.w[d];
-    const double ur = f.lo[d] + (double)(i + 1) * f.w[d];
-    fr[d] = (v - ll) / (ur - ll);
+    fr[d] = (v - ll) * f.iw[d];
   }
   // the four (x, y) corner columns: six floats each (the z pair), all
   // loads issued before any is used, so one memory latency per lookup
@@ -142,14 +142,13 @@ KFD_INLINE void fieldAtGrad(const DevField &f, const double x[3], double B[3], d
       v = f.hi[d];
       clamped = true;
     }
-    int i = (int)floor((v - f.lo[d]) / f.w[d]);
+    int i = (int)floor((v - f.lo[d]) * f.iw[d]);
     i = i < 0 ? 0 : i;
     i = i > f.n[d] - 2 ? f.n[d] - 2 : i;
     idx[d] = i;
     const double ll = f.lo[d] + (double)i * f.w[d];
-    const double ur = f.lo[d] + (double)(i + 1) * f.w[d];
-    fr[d] = (v - ll) / (ur - ll);
-    dfr[d] = clamped ? 0.0 : 1.0 / (ur - ll);
+    fr[d] = (v - ll) * f.iw[d];
+    dfr[d] = clamped ? 0.0 : f.iw[d];
   }
   const double sc = f.scale;
   // every corner value loaded before any is used (one memory latency):

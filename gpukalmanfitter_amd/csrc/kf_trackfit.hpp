@@ -633,7 +633,8 @@ struct TrackFit {
     const double nr[3] = {pl.r[2], pl.r[5], pl.r[8]};
     // PlaneSurface::derivativeFactors, PlaneSurface.ipp:191-204
     const double dp = nr[0] * dir[0] + nr[1] * dir[1] + nr[2] * dir[2];
-    const double nv[3] = {nr[0] / dp, nr[1] / dp, nr[2] / dp};
+    const double idp = 1.0 / dp;
+    const double nv[3] = {nr[0] * idp, nr[1] * idp, nr[2] * idp};
     double sVec[6];
 #pragma unroll
     for (int c = 0; c < 5; ++c)
@@ -722,10 +723,10 @@ struct TrackFit {
     for (int k = 0; k < 3; ++k) pos[k] = (pl.r[3 * k] * par[0] + pl.r[3 * k + 1] * par[1]) + pl.c[k];
     const double cT = cos(par[3]), sT = sin(par[3]);
     const double d0 = cos(par[2]) * sT, d1 = sin(par[2]) * sT, d2 = cT;
-    const double dn = sqrt(d0 * d0 + d1 * d1 + d2 * d2);
-    dir[0] = d0 / dn;
-    dir[1] = d1 / dn;
-    dir[2] = d2 / dn;
+    const double idn = 1.0 / sqrt(d0 * d0 + d1 * d1 + d2 * d2);
+    dir[0] = d0 * idn;
+    dir[1] = d1 * idn;
+    dir[2] = d2 * idn;
     moved();
     p = fabs(1.0 / par[4]);
     t = par[5];
@@ -985,7 +986,8 @@ struct TrackFit {
     const double S00 = cov[sidx(0, 0)] + V00, S01 = cov[sidx(0, 1)] + V01;
     const double S10 = cov[sidx(1, 0)] + V01, S11 = cov[sidx(1, 1)] + V11;
     const double det = S00 * S11 - S01 * S10;
-    const double i00 = S11 / det, i01 = -S01 / det, i10 = -S10 / det, i11 = S00 / det;
+    const double idet = 1.0 / det;
+    const double i00 = S11 * idet, i01 = -S01 * idet, i10 = -S10 * idet, i11 = S00 * idet;
     const double r0 = m0 - xp[0], r1 = m1 - xp[1];
     const double chi2 = r0 * (i00 * r0 + i01 * r1) + r1 * (i10 * r0 + i11 * r1);
     double xf[6];

@@ -64,6 +64,7 @@ struct Host {
       a.field.lo[k] = f->lo[k];
       a.field.hi[k] = f->hi[k];
       a.field.w[k] = f->grid ? (f->hi[k] - f->lo[k]) / (f->n[k] - 1) : 0.0;
+      a.field.iw[k] = f->grid ? 1.0 / a.field.w[k] : 0.0;
     }
     a.field.values = f->values;
     a.smoothing = 0;

@@ -66,11 +66,15 @@ def test_energy_loss_pulls_fit_model(eloss, precision):
 
 @pytest.mark.parametrize("which", ["fit_model", "eloss"])
 def test_fp64_parity_on_fit_model_simulation(fit_model, eloss, which):
-    """The first 5,000 tracks of each sample against the oracle, 1e-9."""
+    """The first 5,000 tracks of each sample against the oracle, 1e-9; the
+    covariance of the loss-only sample, fitted without multiple scattering,
+    within 1e-8: there a one-ulp change of the inputs moves the oracle's own
+    perigee covariance by 3e-9 (tests/test_tolerance_pins.py)."""
     b = (fit_model if which == "fit_model" else eloss).subset(np.arange(5000))
     o_opt = MP.fit_model_options() if which == "fit_model" else MP.eloss_options()
     g = gpu_fit(b, "fp64", **opts(o_opt))
     o, _, _, _ = oracle.fit(b, o_opt)
     assert np.array_equal(g["status"], o.status)
     pd, cd, c2 = compare_fit(g, o)
-    assert max(pd, cd, c2) <= 1e-9, (pd, cd, c2)
+    assert pd <= 1e-9 and c2 <= 1e-9, (pd, c2)
+    assert cd <= (1e-9 if which == "fit_model" else 1e-8), cd

@@ -131,7 +131,9 @@ def test_fp64_option_variants(torch_cuda, opt):
     # C_s = C_f - G (C_p - C_s') G^T cancels almost completely in var(q/p) on
     # long tracks and amplifies the ~1e-15 covariance differences of the
     # forward pass (symmetric storage, J accumulated per step) up to ~1e6 at the
-    # perigee; the oracle's own rounding is amplified alike.
+    # perigee; the oracle's own rounding is amplified alike: one ulp of the
+    # start covariance moves its perigee covariance by 1e-9 to 1e-8
+    # (tests/test_tolerance_pins.py).
     assert cd <= (1e-8 if opt == dict(multiple_scattering=False) else TOL_FP64), cd
 
 
@@ -296,7 +298,9 @@ def test_full_size_config2_properties(torch_cuda):
 # Bar: every fitted parameter within 1e-5 of the FP64 oracle (metric of
 # tests/parity.py), identical status bits.  The smoothed covariance goes
 # through the RTS cancellation C_f - G (C_p - C_s') G^T, which amplifies the
-# float rounding of the stored filtered covariances; its bar is 1e-3.
+# float rounding of the stored filtered covariances; its bar is 1e-3 (float-
+# rounded RTS operands move the oracle's own smoothed covariances by ~1e-4 on
+# 300 tracks: tests/test_tolerance_pins.py).
 TOL_MIXED_PARAMS = 1e-5
 TOL_MIXED_COV = 1e-3
 

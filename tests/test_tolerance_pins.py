@@ -1,0 +1,101 @@
+"""Pins of the tolerances the GPU tests loosen, on the CPU oracle alone: each
+shows that the FP64 oracle itself moves by as much under a perturbation of
+the size the looser bar allows for, so the bar measures the conditioning of
+the computation, not a defect of the kernels.
+
+* Multiple scattering off (covariance bar 1e-8 instead of 1e-9): without
+  scattering noise the RTS step C_s = C_f - G (C_p - C_s') G^T cancels almost
+  completely on long material tracks; a one-ulp change of the start
+  covariance moves the oracle's own perigee covariance by more than 1e-9.
+* The mixed mode's smoothed covariance (bar 1e-3): the mixed kernels store the
+  filtered covariances and Jacobians in float; rounding the RTS operands to
+  float moves the oracle's smoothed covariances by up to ~1e-4, a 1500-fold
+  amplification of float's 6e-8.
+* Rotations in the adjoint's finite differences (step 1e-8 rad, not 1e-6):
+  a 1e-6 rad tilt changes the number of RK steps of some tracks (the
+  navigator's 1e-4 mm on-surface tolerance) and moves the central difference
+  of the chi2 sum by more than the 1e-4 bar; at 1e-8 rad no step count
+  changes and the difference is smooth.
+"""
+import dataclasses
+
+import numpy as np
+
+from gpukalmanfitter_amd import abi, alignment, configs
+from gpukalmanfitter_amd import generate as G
+from gpukalmanfitter_amd.fitter import unpack_sym
+
+import oracle
+from parity import cov_dev
+
+W = configs.WORKLOAD_FIT_OPTIONS
+
+
+def test_ms_off_perigee_covariance_moves_1e9_under_one_ulp(built):
+    b = G.generate(5, n_tracks=4000)
+    b2 = dataclasses.replace(b, start_cov=b.start_cov * (1 + 2.0 ** -52))
+    dev = {}
+    for ms in (1, 0):
+        opt = abi.default_options(multiple_scattering=ms, **W)
+        o1, _, _, _ = oracle.fit(b, opt)
+        o2, _, _, _ = oracle.fit(b2, opt)
+        m = (o1.status & abi.KF_STATUS_FINISHED) != 0
+        dev[ms] = cov_dev(o2.cov[:, m], o1.cov[:, m]).max()
+    print("perigee covariance moved by one ulp of the start covariance:", dev)
+    assert dev[1] < 5e-10          # with scattering: inside the 1e-9 bar
+    assert 1e-9 < dev[0] < 1e-8    # without: beyond 1e-9, inside the 1e-8 bar
+
+
+def test_float_rts_operands_move_smoothed_covariance(built):
+    """The oracle's RTS smoother (kfo_rts_smooth) on its own track states,
+    once with the exact operands (reproduces its smoothed covariances to
+    rounding) and once with the filtered/predicted covariances and Jacobians
+    rounded to float, as the mixed kernels store them."""
+    b = G.generate(3, n_tracks=300)
+    o, _, _, _ = oracle.fit(b, abi.default_options(**W), track_states=True)
+    f32 = lambda x: x.astype(np.float32).astype(np.float64)  # noqa: E731
+    worst_exact = worst_float = 0.0
+    for i in range(b.n_tracks):
+        n = int(o.n_measured[i])
+        if n < 2 or not o.status[i] & abi.KF_STATUS_SMOOTHED:
+            continue
+        rec = o.track_states[:n, :, i]
+        par = lambda off: rec[:, off:off + 6]  # noqa: E731
+        cov = lambda off: np.array([unpack_sym(rec[k, off + 6:off + 27]) for k in range(n)])  # noqa: E731
+        fcov, pcov, scov = cov(abi.KF_TS_FILTERED), cov(abi.KF_TS_PREDICTED), cov(abi.KF_TS_SMOOTHED)
+        jac = rec[:, abi.KF_TS_JACOBIAN:abi.KF_TS_JACOBIAN + 36].reshape(n, 6, 6)
+        d = np.sqrt(np.einsum("kii->ki", scov))
+        norm = d[:, :, None] * d[:, None, :]
+        _, _, sc = oracle.rts_smooth(par(abi.KF_TS_FILTERED), fcov, par(abi.KF_TS_PREDICTED), pcov,
+                                     jac)
+        worst_exact = max(worst_exact, (np.abs(sc - scov) / norm).max())
+        _, _, sc = oracle.rts_smooth(par(abi.KF_TS_FILTERED), f32(fcov), par(abi.KF_TS_PREDICTED),
+                                     f32(pcov), f32(jac))
+        worst_float = max(worst_float, (np.abs(sc - scov) / norm).max())
+    print(f"smoothed covariance: exact operands {worst_exact:.3g}, float operands {worst_float:.3g}")
+    assert worst_exact < 1e-9
+    assert 1e-5 < worst_float < 1e-3
+
+
+def test_rotation_fd_step_of_1e6_flips_step_counts(built):
+    b = G.generate(4, n_tracks=20000)
+    opt = abi.default_options(**dict(W, multiple_scattering=False, energy_loss=False))
+    npl = len(b.planes)
+
+    def chi2_steps(d):
+        o, steps, _, _ = oracle.fit(b, opt, planes=alignment.apply(b.planes, d))
+        return o.chi2.sum(), steps
+
+    _, s0 = chi2_steps(np.zeros((npl, 6)))
+    fd, flips = {}, {}
+    for h in (1e-6, 1e-8):
+        d = np.zeros((npl, 6))
+        d[7, 5] = h  # rotation about the normal of plane 7
+        cp, sp = chi2_steps(d)
+        cm, sm = chi2_steps(-d)
+        fd[h] = (cp - cm) / (2 * h)
+        flips[h] = int(((sp != s0) | (sm != s0)).sum())
+    rel = abs(fd[1e-6] - fd[1e-8]) / abs(fd[1e-8])
+    print("step-count changes", flips, "relative FD difference", rel)
+    assert flips[1e-6] > 0 and flips[1e-8] == 0
+    assert rel > 1e-4
