@@ -6,6 +6,7 @@ calling it is done by :mod:`gpukalmanfitter_amd.fitter`.
 import ctypes as C
 
 KF_ABI_VERSION = 3
+KF_NOALLOC_WORKSPACE_BYTES = 256 << 20
 
 KF_OK = 0
 KF_E_INVALID = -1

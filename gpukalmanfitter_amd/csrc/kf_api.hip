@@ -11,6 +11,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <vector>
 
@@ -18,6 +19,13 @@
 #include "kf_adjoint.hpp"
 #include "kf_kernels.hpp"
 #include "kf_setup.hpp"
+
+// The fixed workspace of kf_fit_noalloc: one copy on every device the library
+// runs on, held until the process ends.  Host code never uses this name as an
+// address (on the host it is only a stand-in for the device symbol): the
+// device address comes from hipGetSymbolAddress, once per device, in
+// kf_ctx_create (noallocWorkspace below).
+static __device__ __align__(256) unsigned char scratch[KF_NOALLOC_WORKSPACE_BYTES];
 
 struct kf_ctx {
   int device = 0;
@@ -57,6 +65,34 @@ struct kf_ctx {
 namespace {
 
 char g_create_err[512] = {0};
+
+// Device addresses of `scratch`, per device (resolved with that device
+// current); g_ws_state: 0 unresolved, 1 resolved, < 0 the hipError_t of a
+// failed resolution, negated.
+constexpr int kMaxDevices = 64;
+std::mutex g_ws_mu;
+void *g_ws_ptr[kMaxDevices] = {};
+int g_ws_state[kMaxDevices] = {};
+
+// Resolves the workspace of `device` (the current device) once.
+hipError_t noallocWorkspace(int device, void **out) {
+  *out = nullptr;
+  if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(g_ws_mu);
+  if (g_ws_state[device] == 0) {
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      g_ws_state[device] = -(int)((e != hipSuccess) ? e : hipErrorInvalidSymbol);
+    } else {
+      g_ws_ptr[device] = p;
+      g_ws_state[device] = 1;
+    }
+  }
+  if (g_ws_state[device] < 0) return (hipError_t)(-g_ws_state[device]);
+  *out = g_ws_ptr[device];
+  return hipSuccess;
+}
 
 int fail(kf_ctx *ctx, int code, const char *fmt, ...) {
   char *buf = ctx ? ctx->err : g_create_err;
@@ -140,6 +176,10 @@ int kf_ctx_create(int device, kf_ctx **out) {
   if (e == hipSuccess) e = hipEventCreate(&ctx->evm);
   if (e == hipSuccess) e = hipDeviceGetAttribute(&ctx->n_cu, hipDeviceAttributeMultiprocessorCount, device);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->cap_stream, hipStreamNonBlocking);
+  if (e == hipSuccess) {
+    void *ws = nullptr;
+    e = noallocWorkspace(device, &ws);
+  }
   kf_schedule_default(&ctx->sched);
   if (e != hipSuccess) {
     fail(nullptr, KF_E_HIP, "kf_ctx_create: %s", hipGetErrorString(e));
@@ -423,6 +463,58 @@ int prepareFit(kf_ctx *ctx, const kf_tracks *tr, const kf_options *opt, const kf
 }  // namespace
 
 extern "C" {
+
+int kf_fit_noalloc(kf_ctx *ctx, const kf_tracks *tr, const kf_options *opt,
+                   const kf_results *res, void *stream) {
+  kfd::FitArgs a;
+  int rc = prepareFit(ctx, tr, opt, res, "kf_fit_noalloc", a);
+  if (rc != KF_OK || tr->n_tracks == 0) return rc;
+  void *ws = nullptr;
+  const hipError_t we = noallocWorkspace(ctx->device, &ws);
+  if (we != hipSuccess)
+    return fail(ctx, KF_E_HIP, "kf_fit_noalloc: workspace of device %d unavailable: %s",
+                ctx->device, hipGetErrorString(we));
+  const bool ts = res->track_states != nullptr;
+  // the largest multiple of 64 positions whose storage fits the workspace
+  auto fits = [&](int64_t slice) {
+    const StoreLayout L = storeLayout(opt->precision, ts, false, slice, a.max_states);
+    return L.store() + L.carry_bytes <= (size_t)KF_NOALLOC_WORKSPACE_BYTES;
+  };
+  if (!fits(64)) {
+    const StoreLayout L = storeLayout(opt->precision, ts, false, 64, a.max_states);
+    return fail(ctx, KF_E_INVALID,
+                "kf_fit_noalloc: one 64-track slice with %d states per track needs %zu bytes, "
+                "the fixed workspace holds %llu (lower max_states)",
+                a.max_states, L.store() + L.carry_bytes,
+                (unsigned long long)KF_NOALLOC_WORKSPACE_BYTES);
+  }
+  int64_t lo = 64, hi = (a.n + 63) / 64 * 64;
+  if (fits(hi)) {
+    lo = hi;
+  } else {
+    while (hi - lo > 64) {  // fits(lo), !fits(hi)
+      const int64_t mid = (lo + hi) / 128 * 64;
+      if (fits(mid))
+        lo = mid;
+      else
+        hi = mid;
+    }
+  }
+  const int64_t slice = lo >= a.n ? a.n : lo;
+  const StoreLayout L = storeLayout(opt->precision, ts, false, slice, a.max_states);
+  carveStores(L, ws, (char *)ws + L.store(), ts, false, a);
+  hipStream_t s = (hipStream_t)stream;
+  KfdLaunch launch{};  // one thread per track, identity order
+  KF_HIP(ctx, hipEventRecord(ctx->ev0, s));
+  const hipError_t e = launchSlices(a, launch, opt->precision, res->track_states, slice,
+                                    ctx->evm, s);
+  if (e != hipSuccess)
+    return fail(ctx, KF_E_HIP, "kf_fit_noalloc: launch failed: %s", hipGetErrorString(e));
+  KF_HIP(ctx, hipEventRecord(ctx->ev1, s));
+  ctx->timed = true;
+  ctx->phase_timed = slice >= a.n;
+  return KF_OK;
+}
 
 int kf_fit(kf_ctx *ctx, const kf_tracks *tr, const kf_options *opt, const kf_results *res,
            void *stream) {

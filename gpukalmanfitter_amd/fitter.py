@@ -54,6 +54,9 @@ def _lib():
         lib.kf_last_kernel_ms.restype = C.c_double
         lib.kf_last_phase_ms.argtypes = [P, P, P]
         lib.kf_last_phase_ms.restype = C.c_int
+        lib.kf_fit_noalloc.argtypes = [P, C.POINTER(abi.kf_tracks), C.POINTER(abi.kf_options),
+                                       C.POINTER(abi.kf_results), P]
+        lib.kf_fit_noalloc.restype = C.c_int
         lib.kf_graph_captures.argtypes = [P]
         lib.kf_graph_captures.restype = C.c_int64
         lib.kf_chi2_gradient.argtypes = [P, C.POINTER(abi.kf_tracks), C.POINTER(abi.kf_options),
@@ -317,6 +320,20 @@ class KalmanFitter:
         s = stream if stream is not None else torch.cuda.current_stream(dev_batch.device)
         self._check(self._lib.kf_fit(self._ctx, C.byref(tracks), C.byref(opts), C.byref(results),
                                      C.c_void_p(s.cuda_stream)))
+
+    def fit_batch_noalloc(self, dev_batch, options=None, stream=None):
+        """fit_batch through kf_fit_noalloc: no device allocation, the library's
+        fixed per-device workspace instead (one thread per track; calls that
+        overlap on one GPU race)."""
+        torch = _torch()
+        opts = (options or KalmanFitterOptions()).to_c()
+        if opts.max_states == 0:
+            opts.max_states = dev_batch.max_states
+        tracks = dev_batch.c_tracks()
+        results = dev_batch.c_results()
+        s = stream if stream is not None else torch.cuda.current_stream(dev_batch.device)
+        self._check(self._lib.kf_fit_noalloc(self._ctx, C.byref(tracks), C.byref(opts),
+                                             C.byref(results), C.c_void_p(s.cuda_stream)))
 
     def set_schedule(self, **fields):
         """Sets kf_schedule fields (unnamed ones keep the library defaults)."""

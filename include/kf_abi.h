@@ -210,6 +210,27 @@ int kf_set_field_scale(kf_ctx *ctx, double s);
 int kf_fit(kf_ctx *ctx, const kf_tracks *tracks, const kf_options *opt,
            const kf_results *res, void *stream);
 
+/* kf_fit without any device allocation.  No counterpart in the reference:
+ * for callers that cannot allocate (code under HIP graph capture, serving
+ * loops, harnesses that forbid hipMalloc).  The working storage (smoother
+ * store, hand-over records, track-state staging) comes from a fixed
+ * device-resident workspace of KF_NOALLOC_WORKSPACE_BYTES that the library
+ * holds on every device it runs on until the process ends; batches larger
+ * than it holds run in consecutive slices of 64-track multiples, with the
+ * same results as kf_fit bit for bit.  Always the one-thread-per-track
+ * schedule (no length sort, work queue or graph replay).  Returns
+ * KF_E_INVALID when the working storage of one 64-track slice,
+ * 64 x (max_states x (512 B in FP64, 304 B mixed, + 960 B with track
+ * states) + 264 B), does not fit the workspace (max_states: opt->max_states,
+ * else the plane-table size; in FP64 without track states up to 8,191).  Calls on
+ * one device share its workspace: calls that overlap in time on one GPU
+ * (different streams or host threads) race and must be serialised by the
+ * caller.  The context must have been created (kf_ctx_create resolves the
+ * workspace of its device). */
+#define KF_NOALLOC_WORKSPACE_BYTES (256ull << 20)
+int kf_fit_noalloc(kf_ctx *ctx, const kf_tracks *tracks, const kf_options *opt,
+                   const kf_results *res, void *stream);
+
 /* Gradient of the summed chi2 of the forward filter (the fit's results.chi2
  * summed over the batch) with respect to the alignment of every plane of the
  * table and to the field scale, by a hand-written adjoint (reverse sweep over
