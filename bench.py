@@ -89,6 +89,8 @@ def parse():
                     help="skip the 4,194,304-track material workload leg")
     ap.add_argument("--no-intersect", action="store_true",
                     help="skip the batched ray x triangle intersection leg")
+    ap.add_argument("--no-config1", action="store_true",
+                    help="skip the 10,000-track config-1 leg")
     ap.add_argument("--no-adjoint", action="store_true",
                     help="skip the chi2-adjoint leg (config 4, 262,144 tracks)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -312,6 +314,11 @@ def ragged_leg(steps, warmup, sample, precision="mixed"):
         return (time.perf_counter() - t) / steps
 
     default_s = timed()
+    # lane groups (the counterpart of the reference's block-per-track kernel)
+    lane_rates = {}
+    for lanes in (2, 4):
+        kf.set_schedule(lanes_per_track=lanes, sort_by_length=1, graph=1)
+        lane_rates[lanes] = b.n_tracks / timed()
     best, tune_ms = kf.autotune(dev, opts)
     tuned_s = timed()
     r = dev.results_numpy()
@@ -327,6 +334,8 @@ def ragged_leg(steps, warmup, sample, precision="mixed"):
             "tracks_per_s": b.n_tracks / tuned_s, "ms_per_step": tuned_s * 1e3,
             "schedule": best, "autotune_best_kernel_ms": tune_ms,
             "default_schedule_tracks_per_s": b.n_tracks / default_s,
+            "lane_groups_tracks_per_s": {f"{k} lanes/track, sorted, graph": v
+                                         for k, v in lane_rates.items()},
             "fp64_same_schedule_tracks_per_s": b.n_tracks / fp64_s,
             "mixed_over_fp64": fp64_s / tuned_s}, b.subset(np.arange(min(sample, b.n_tracks))), res
 
@@ -334,8 +343,9 @@ def ragged_leg(steps, warmup, sample, precision="mixed"):
 def material_leg(steps, warmup, sample):
     """The material workload: 4,194,304 tracks over 8-40 planes of 300 um
     silicon (Highland scattering, Bethe energy loss) in the grid field.  The
-    FP32-mixed fit with the autotuned schedule, the FP64 fit with the same
-    schedule, and the chi2 adjoint with respect to 6 x 40 alignment parameters
+    FP32-mixed and FP64 fits, each with the schedule the autotuner picks for
+    it (and with 2 and 4 lanes per track beside it), and the chi2 adjoint
+    with respect to 6 x 40 alignment parameters
     and the field scale (material differentiated).  Returns the figures, the
     first `sample` tracks and both fits' results on them (checked against
     the oracle in the CPU leg).  The working set exceeds the library's
@@ -388,11 +398,16 @@ def material_leg(steps, warmup, sample):
                                  "device sync for the hit count"}}
     res = {}
     rates = {}
+    out["schedule"] = {}
+    out["lane_groups_tracks_per_s"] = {}
     for prec in ("mixed", "fp64"):
         opts = KalmanFitterOptions(precision=prec, **configs.WORKLOAD_FIT_OPTIONS)
-        if prec == "mixed":
-            best, _ = kf.autotune(dev, opts)
-            out["schedule"] = best
+        for lanes in (2, 4):
+            kf.set_schedule(lanes_per_track=lanes, sort_by_length=1, graph=1)
+            out["lane_groups_tracks_per_s"][f"{prec}, {lanes} lanes/track, sorted, graph"] = \
+                b.n_tracks / timed(lambda: kf.fit_batch(dev, opts))
+        best, _ = kf.autotune(dev, opts)
+        out["schedule"][prec] = best
         sec = timed(lambda: kf.fit_batch(dev, opts))
         rates[prec] = b.n_tracks / sec
         r = dev.results_numpy()
@@ -625,8 +640,10 @@ def main():
           f"{e2e['ms_per_step']:.2f} ms/step, {e2e['pcie_gb_per_s']:.1f} GB/s over PCIe",
           file=sys.stderr, flush=True)
     others = {}
-    c1, c1b, c1r = config1_leg(max(args.steps // 2, 3))
-    others["config1"] = (c1b, c1r)
+    c1 = None
+    if not args.no_config1:
+        c1, c1b, c1r = config1_leg(max(args.steps // 2, 3))
+        others["config1"] = (c1b, c1r)
     ragged = None
     if not args.no_ragged:
         ragged, rsub, rres = ragged_leg(max(args.steps // 2, 3), 1, 10000)
@@ -660,7 +677,8 @@ def main():
             ragged["error_vs_oracle"] = oerr["config3"]
         if adj is not None:
             adj["forward_fit_error_vs_oracle"] = oerr["config4"]
-        c1["error_vs_oracle"] = oerr["config1"]
+        if c1 is not None:
+            c1["error_vs_oracle"] = oerr["config1"]
         summary = {"config2": errors}
         summary.update({{"material": "config5"}.get(k, k): v for k, v in oerr.items()})
         error_by_config = {

@@ -528,6 +528,7 @@ int kf_fit(kf_ctx *ctx, const kf_tracks *tr, const kf_options *opt, const kf_res
   if (rc != KF_OK) return rc;
   const kf_schedule &sc = ctx->sched;
   KfdLaunch L{};
+  L.lanes_per_track = sc.lanes_per_track;
   if (sc.sort_by_length || sc.persistent) {
     const size_t order_bytes = ((size_t)a.n * sizeof(int32_t) + 255) / 256 * 256;
     rc = grow(ctx, &ctx->sched_buf, &ctx->sched_bytes, order_bytes + 514 * sizeof(uint32_t), s,
@@ -607,8 +608,11 @@ void kf_schedule_default(kf_schedule *s) {
 int kf_set_schedule(kf_ctx *ctx, const kf_schedule *s) {
   if (!ctx) return fail(nullptr, KF_E_INVALID, "kf_set_schedule: ctx is NULL");
   if (!s) return fail(ctx, KF_E_INVALID, "kf_set_schedule: schedule is NULL");
-  if (s->lanes_per_track != 1)
-    return fail(ctx, KF_E_INVALID, "kf_set_schedule: lanes_per_track must be 1 (one track per lane)");
+  if (s->lanes_per_track != 1 && s->lanes_per_track != 2 && s->lanes_per_track != 4)
+    return fail(ctx, KF_E_INVALID, "kf_set_schedule: lanes_per_track must be 1, 2 or 4");
+  if (s->lanes_per_track > 1 && s->persistent)
+    return fail(ctx, KF_E_INVALID,
+                "kf_set_schedule: lanes_per_track > 1 runs on the one-shot grid (persistent = 0)");
   if (s->persistent && (s->waves_per_cu < 1 || s->waves_per_cu > 64 || s->chunks_per_grab < 1 ||
                         s->chunks_per_grab > 1024))
     return fail(ctx, KF_E_INVALID, "kf_set_schedule: waves_per_cu in [1, 64], chunks_per_grab in [1, 1024]");
@@ -634,6 +638,11 @@ int kf_autotune(kf_ctx *ctx, const kf_tracks *tracks, const kf_options *opt, con
     c.sort_by_length = sort;
     c.graph = 1;
     cand.push_back(c);
+    for (int l : {2, 4}) {  // lane groups (one-shot grid)
+      kf_schedule cl = c;
+      cl.lanes_per_track = l;
+      cand.push_back(cl);
+    }
     for (int w : {2, 4, 8})
       for (int g : {1, 4}) {
         c.persistent = 1;

@@ -32,6 +32,63 @@ namespace kfd {
 // point (bounds the number of live registers in load-heavy phases).
 #define KFD_PHASE_FENCE() __asm__ volatile("" ::: "memory")
 
+// --------------------------------------------------------- lane groups ----
+// A track may be fitted by a group of L = 1, 2 or 4 consecutive lanes of a
+// quad (the counterpart of the reference's block-per-track kernel): every
+// lane of a group runs the track's program on the same data, so a group never
+// diverges, and the work of a few matrix products is split by rank (the lane's
+// index in its group) and gathered back.  groupGet<L, SRC>(v) is the v of
+// rank SRC of the caller's group: one DPP quad_perm move per 32 bits.
+template <int L, int SRC>
+KFD_INLINE int groupGet32(int v) {
+  static_assert(L == 1 || L == 2 || L == 4, "lane groups of 1, 2 or 4 lanes");
+  static_assert(SRC >= 0 && SRC < L, "source rank inside the group");
+  if constexpr (L == 1) {
+    return v;
+  } else {
+#if defined(__HIP_DEVICE_COMPILE__)
+    // quad_perm: lane j of a quad reads lane perm[j]; groups of two lanes
+    // are (0, 1) and (2, 3)
+    constexpr int base2 = SRC | (SRC << 2) | ((2 | SRC) << 4) | ((2 | SRC) << 6);
+    constexpr int ctrl = (L == 2) ? base2 : SRC * 0x55;
+    return __builtin_amdgcn_update_dpp(v, v, ctrl, 0xF, 0xF, false);
+#else
+    return v;  // host build: groups of one lane only
+#endif
+  }
+}
+template <int L, int SRC>
+KFD_INLINE float groupGet(float v) {
+  return __builtin_bit_cast(float, groupGet32<L, SRC>(__builtin_bit_cast(int, v)));
+}
+template <int L, int SRC>
+KFD_INLINE double groupGet(double v) {
+  const uint64_t u = __builtin_bit_cast(uint64_t, v);
+  const int lo = groupGet32<L, SRC>((int)(uint32_t)u);
+  const int hi = groupGet32<L, SRC>((int)(uint32_t)(u >> 32));
+  return __builtin_bit_cast(double, ((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo);
+}
+// out[c] = the value of slot c / L of rank c % L, for c < N: the gather of N
+// values computed in slots (rank r holds values r, r + L, r + 2L, ... in
+// slot[0], slot[1], ...).
+template <int L, int N, typename T>
+KFD_INLINE void groupGather(const T *slot, T *out) {
+#pragma unroll
+  for (int c = 0; c < N; ++c) {
+    if constexpr (L == 1) {
+      out[c] = slot[c];
+    } else if (c % L == 0) {
+      out[c] = groupGet<L, 0>(slot[c / L]);
+    } else if (c % L == 1) {
+      out[c] = groupGet<L, (1 < L) ? 1 : 0>(slot[c / L]);
+    } else if (c % L == 2) {
+      out[c] = groupGet<L, (2 < L) ? 2 : 0>(slot[c / L]);
+    } else {
+      out[c] = groupGet<L, (3 < L) ? 3 : 0>(slot[c / L]);
+    }
+  }
+}
+
 constexpr double kOnSurfaceTolerance = 1e-4;
 constexpr double kOverstepLimit = 0.01;
 constexpr double kPi = 3.14159265358979323846;

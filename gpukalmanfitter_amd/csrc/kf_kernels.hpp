@@ -146,6 +146,7 @@ extern "C" hipError_t kfd_launch_intersect(const kf_polygon_plane *surfaces, int
 struct KfdLaunch {
   int32_t sort_by_length;  // order positions by decreasing sequence length first
   int32_t persistent_blocks;  // > 0: persistent grid of this many blocks
+  int32_t lanes_per_track;  // 1, 2 or 4 lanes per position (one-shot grid only)
   uint32_t *sort_scratch;  // 256 + 256 counters (length histogram, bin offsets)
   int32_t *order;          // [n] positions -> tracks (when sorting)
 };

@@ -16,7 +16,9 @@
 //   -g,--grid-size, -b,--block-size
 //                          accepted for compatibility; the launch shape is
 //                          one track per lane, 128-lane blocks
-//   -s,--shared-memory 1   refused (no block-per-track variant)
+//   -s,--shared-memory 1   the counterpart of the reference's block-per-track
+//                          kernel: each track on a group of 4 lanes
+//                          (kf_schedule.lanes_per_track = 4)
 //   -m,--smoothing 0|1     run the smoother (default 1)
 //   -a,--machine NAME      label of the timing line
 // Options of this build:
@@ -131,7 +133,7 @@ void usage() {
                "Usage: kf_run [options]\n"
                "\t-t,--tracks N\n\t-e,--streams N\n\t-r,--threads N\n"
                "\t-u,--multiple-devices 0\n\t-o,--output 0|1\n\t-d,--device gpu\n"
-               "\t-g,--grid-size X*Y\n\t-b,--block-size X*Y*Z\n\t-s,--shared-memory 0\n"
+               "\t-g,--grid-size X*Y\n\t-b,--block-size X*Y*Z\n\t-s,--shared-memory 0|1\n"
                "\t-m,--smoothing 0|1\n\t-a,--machine NAME\n\t-c,--config 1|2|3|5\n"
                "\t-p,--precision fp64|mixed\n\t-n,--repeat N\n\t--reference-overstep\n"
                "\t--with-copies\n\t--per-track K\n"
@@ -167,7 +169,7 @@ void toHost(std::vector<T> &h, const T *d) {
 
 int main(int argc, char **argv) {
   int64_t nTracks = -1;
-  int nStreams = 1, nThreads = 0, config = 1, repeat = 10;
+  int nStreams = 1, nThreads = 0, config = 1, repeat = 10, lanesPerTrack = 1;
   int64_t perTrack = 0;
   bool output = false, smoothing = true, refOverstep = false, printConfig = false;
   bool withCopies = false;
@@ -203,10 +205,7 @@ int main(int argc, char **argv) {
       const std::string v = next("--grid-size/--block-size");
       std::printf("INFO: %s %s ignored: one track per lane, 128-lane blocks\n", a.c_str(), v.c_str());
     } else if (a == "-s" || a == "--shared-memory") {
-      if (std::atoi(next("--shared-memory").c_str()) == 1) {
-        std::fprintf(stderr, "ERROR: no block-per-track variant in this build\n");
-        return 1;
-      }
+      lanesPerTrack = std::atoi(next("--shared-memory").c_str()) == 1 ? 4 : 1;
     } else if (a == "-m" || a == "--smoothing") {
       smoothing = std::atoi(next("--smoothing").c_str()) == 1;
     } else if (a == "-a" || a == "--machine") {
@@ -381,6 +380,15 @@ int main(int argc, char **argv) {
       return 1;
     }
     fitters.emplace_back(new kfmi::KalmanFitter(planes, field, 0));
+    if (lanesPerTrack > 1) {
+      kf_schedule sched;
+      kf_schedule_default(&sched);
+      sched.lanes_per_track = lanesPerTrack;
+      if (kf_set_schedule(fitters.back()->context(), &sched) != KF_OK) {
+        std::fprintf(stderr, "ERROR: %s\n", kf_last_error(fitters.back()->context()));
+        return 1;
+      }
+    }
   }
   kfmi::KalmanFitterOptions opt;
   opt.smoothing = smoothing;
@@ -519,9 +527,9 @@ int main(int argc, char **argv) {
   std::sort(ms.begin(), ms.end());
   const double med = ms[ms.size() / 2];
   std::printf("INFO: Time (ms) for KF track fitting on device 0 : %f\n", med);
-  std::printf("INFO: %s %.4g tracks/s (median of %d fits, %d stream(s)%s)\n",
+  std::printf("INFO: %s %.4g tracks/s (median of %d fits, %d stream(s), %d lane(s) per track%s)\n",
               machine.empty() ? "MI355X" : machine.c_str(), n / (med * 1e-3), repeat, nStreams,
-              withCopies ? ", inputs copied in and results out in the timed region" : "");
+              lanesPerTrack, withCopies ? ", inputs copied in and results out in the timed region" : "");
   if (copyFailed) {
     std::fprintf(stderr, "ERROR: hipMemcpyAsync failed (--with-copies)\n");
     return 1;

@@ -25,11 +25,23 @@ namespace kfd {
 // the perigee) that resumes from that record.
 enum { kResumeNone = 0, kResumePreLoop = 1, kResumeInLoop = 2, kResumePostLoop = 3 };
 
-template <typename S, bool FWD>
+// L: lanes per track (1, 2 or 4; kf_device.hpp, lane groups).  The lanes of a
+// group run this program on the same data; the free transport matrix of each
+// RK step is built column by column across them, and only the group's first
+// lane writes to memory.
+template <typename S, bool FWD, int L = 1>
 struct TrackFit {
   static constexpr bool kMixed = sizeof(S) < sizeof(double);
   const FitArgs &a;
   const int64_t i;
+  // index of this lane in its group, and whether it writes the track's
+  // records and outputs
+  uint32_t rank = 0;
+  bool lead = true;
+  KFD_INLINE void setRank(uint32_t r) {
+    rank = r;
+    lead = (r == 0);
+  }
 
   // ---- stepper state (EigenStepper::State, EigenStepper.hpp:27-120) ----
   double pos[3], dir[3], p, t, q;
@@ -162,6 +174,7 @@ struct TrackFit {
   }
   // six parameters at tsd element f (0 or 6) of state k
   KFD_INLINE void storePar(int k, int f, const double v[6]) const {
+    if (!lead) return;
 #pragma unroll
     for (int g = 0; g < 6; g += 2) st16(grpD(k, f + g), v + g);
   }
@@ -172,6 +185,7 @@ struct TrackFit {
   // n elements (a multiple of kW) at tss element f of state k
   template <int N>
   KFD_INLINE void storeS(int k, int f, const S v[N]) const {
+    if (!lead) return;
 #pragma unroll
     for (int g = 0; g < N; g += kW) st16(grpS(k, f + g), v + g);
   }
@@ -185,6 +199,7 @@ struct TrackFit {
   // Checkpoint k: the state right after the update and material of measured
   // surface k, from which the adjoint recomputes the next segment.
   KFD_INLINE void saveCheckpoint(int k) const {
+    if (!lead) return;
 #pragma unroll
     for (int m = 0; m < 3; ++m) {
       *recC(k, KF_CK_POS + m) = pos[m];
@@ -242,6 +257,7 @@ struct TrackFit {
   // public record: parameters + covariance at offset off
   template <typename T>
   KFD_INLINE void storeBoundF(int k, int off, const double par[6], const T c[21]) const {
+    if (!lead) return;
 #pragma unroll
     for (int m = 0; m < 6; ++m) *recF(k, off + m) = par[m];
 #pragma unroll
@@ -483,30 +499,91 @@ struct TrackFit {
 #pragma unroll
     for (int k = 0; k < 3; ++k) dk4dL[k] = c1[k] + (qop * h) * c2[k];
 
-    // dk/dT matrices (row r, column c)
-    S dk1dT[3][3] = {{(S)0, Bf[2], -Bf[1]}, {-Bf[2], (S)0, Bf[0]}, {Bf[1], -Bf[0], (S)0}};
-#pragma unroll
-    for (int r = 0; r < 3; ++r)
-#pragma unroll
-      for (int c = 0; c < 3; ++c) dk1dT[r][c] = qop * dk1dT[r][c];
-    S dk2dT[3][3], dk3dT[3][3], dk4dT[3][3];
-    crossStep(dk1dT, half_h, Bm, qop, dk2dT);
-    crossStep(dk2dT, half_h, Bm, qop, dk3dT);
-    crossStep(dk3dT, h, Bl, qop, dk4dT);
-
     // divisions: a float 1/6 constant is biased by 3e-8 relative, which
     // accumulates over the steps of a track
     const S h6 = h / (S)6.0;
     const S hh6 = (h * h) / (S)6.0;
     S dFdT[3][3], dGdT[3][3], dFdL[3], dGdL[3];
+    if constexpr (L == 1) {
+      // dk/dT matrices (row r, column c)
+      S dk1dT[3][3] = {{(S)0, Bf[2], -Bf[1]}, {-Bf[2], (S)0, Bf[0]}, {Bf[1], -Bf[0], (S)0}};
+#pragma unroll
+      for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) dk1dT[r][c] = qop * dk1dT[r][c];
+      S dk2dT[3][3], dk3dT[3][3], dk4dT[3][3];
+      crossStep(dk1dT, half_h, Bm, qop, dk2dT);
+      crossStep(dk2dT, half_h, Bm, qop, dk3dT);
+      crossStep(dk3dT, h, Bl, qop, dk4dT);
+#pragma unroll
+      for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          const S id = (r == c) ? (S)1 : (S)0;
+          dFdT[r][c] = h * (id + h6 * (dk1dT[r][c] + dk2dT[r][c] + dk3dT[r][c]));
+          dGdT[r][c] =
+              id + h6 * (dk1dT[r][c] + (S)2.0 * (dk2dT[r][c] + dk3dT[r][c]) + dk4dT[r][c]);
+        }
+    } else {
+      // lane groups: the columns of dF/dT and dG/dT are independent (column
+      // c follows the chain dk1..dk4 of the unit vector e_c); each lane
+      // builds the columns of its slots, c = rank + L * slot (the last
+      // column again where that runs past it), and the group gathers them
+      constexpr int kSlots = (3 + L - 1) / L;
+      S colF[kSlots][3], colG[kSlots][3];
+#pragma unroll
+      for (int sl = 0; sl < kSlots; ++sl) {
+        const uint32_t c = (rank + (uint32_t)(L * sl) < 3u) ? rank + (uint32_t)(L * sl) : 2u;
+        const S e[3] = {(c == 0u) ? (S)1 : (S)0, (c == 1u) ? (S)1 : (S)0,
+                        (c == 2u) ? (S)1 : (S)0};
+        S k1c[3], k2c[3], k3c[3], k4c[3], col[3];
+        // column c of dk1/dT = qop [B]x e_c: the unit-vector cross product
+        // spelled out as the matrix of the single-lane path
+        const S m1[3] = {e[1] * Bf[2] - e[2] * Bf[1], e[2] * Bf[0] - e[0] * Bf[2],
+                         e[0] * Bf[1] - e[1] * Bf[0]};
+#pragma unroll
+        for (int k = 0; k < 3; ++k) k1c[k] = qop * m1[k];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) col[k] = e[k] + half_h * k1c[k];
+        cross(col, Bm, k2c);
+#pragma unroll
+        for (int k = 0; k < 3; ++k) k2c[k] = qop * k2c[k];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) col[k] = e[k] + half_h * k2c[k];
+        cross(col, Bm, k3c);
+#pragma unroll
+        for (int k = 0; k < 3; ++k) k3c[k] = qop * k3c[k];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) col[k] = e[k] + h * k3c[k];
+        cross(col, Bl, k4c);
+#pragma unroll
+        for (int k = 0; k < 3; ++k) k4c[k] = qop * k4c[k];
+#pragma unroll
+        for (int r = 0; r < 3; ++r) {
+          colF[sl][r] = h * (e[r] + h6 * (k1c[r] + k2c[r] + k3c[r]));
+          colG[sl][r] = e[r] + h6 * (k1c[r] + (S)2.0 * (k2c[r] + k3c[r]) + k4c[r]);
+        }
+      }
+      // gather: column c from rank c % L, slot c / L
+#pragma unroll
+      for (int r = 0; r < 3; ++r) {
+        S f[kSlots], g[kSlots], fo[3], go[3];
+#pragma unroll
+        for (int sl = 0; sl < kSlots; ++sl) {
+          f[sl] = colF[sl][r];
+          g[sl] = colG[sl][r];
+        }
+        groupGather<L, 3>(f, fo);
+        groupGather<L, 3>(g, go);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          dFdT[r][c] = fo[c];
+          dGdT[r][c] = go[c];
+        }
+      }
+    }
 #pragma unroll
     for (int r = 0; r < 3; ++r) {
-#pragma unroll
-      for (int c = 0; c < 3; ++c) {
-        const S id = (r == c) ? (S)1 : (S)0;
-        dFdT[r][c] = h * (id + h6 * (dk1dT[r][c] + dk2dT[r][c] + dk3dT[r][c]));
-        dGdT[r][c] = id + h6 * (dk1dT[r][c] + (S)2.0 * (dk2dT[r][c] + dk3dT[r][c]) + dk4dT[r][c]);
-      }
       dFdL[r] = hh6 * (dk1dL[r] + dk2dL[r] + dk3dL[r]);
       dGdL[r] = h6 * (dk1dL[r] + (S)2.0 * (dk2dL[r] + dk3dL[r]) + dk4dL[r]);
     }
@@ -730,6 +807,55 @@ struct TrackFit {
     moved();
     p = fabs(1.0 / par[4]);
     t = par[5];
+  }
+
+  // sin(d), cos(d) for |d| < 1/16 by their Taylor series to d^9 / d^10 (the
+  // first omitted terms are below 2e-21)
+  KFD_INLINE static void sinCosSmall(double d, double &s, double &c) {
+    const double d2 = d * d;
+    s = d * (1.0 - d2 * (1.0 / 6.0) *
+                       (1.0 - d2 * (1.0 / 20.0) * (1.0 - d2 * (1.0 / 42.0) * (1.0 - d2 * (1.0 / 72.0)))));
+    c = 1.0 - d2 * 0.5 *
+                  (1.0 - d2 * (1.0 / 12.0) *
+                             (1.0 - d2 * (1.0 / 30.0) *
+                                        (1.0 - d2 * (1.0 / 56.0) * (1.0 - d2 * (1.0 / 90.0)))));
+  }
+
+  // updateFromBound after the gain-matrix update on a plane, while dir still
+  // holds the predicted direction: cos and sin of the filtered angles follow
+  // from those of the predicted ones (dir's components) by the angle-addition
+  // theorems with the update d = filtered - predicted, in place of four
+  // library calls; an update of 1/16 rad or more, or a direction within 1e-3
+  // of the z axis, takes updateFromBound.  Agrees with it to a few ulp.
+  KFD_INLINE void updateFromFiltered(const DevPlane &pl, const double xf[6], const double xp[6]) {
+    double dphi = xf[2] - xp[2];  // modulo 2 pi (xf[2] is wrapped)
+    if (dphi > kPi)
+      dphi -= 2.0 * kPi;
+    else if (dphi < -kPi)
+      dphi += 2.0 * kPi;
+    const double dth = xf[3] - xp[3];
+    const double sinT = sqrt(dir[0] * dir[0] + dir[1] * dir[1]), cosT = dir[2];
+    if (!(fabs(dphi) < 0.0625 && fabs(dth) < 0.0625 && sinT > 1e-3)) {
+      updateFromBound(pl, xf);
+      return;
+    }
+    const double iS = 1.0 / sinT;
+    const double cosP = dir[0] * iS, sinP = dir[1] * iS;
+    double sp, cp, st, ct;
+    sinCosSmall(dphi, sp, cp);
+    sinCosSmall(dth, st, ct);
+    const double cPf = cosP * cp - sinP * sp, sPf = sinP * cp + cosP * sp;
+    const double cTf = cosT * ct - sinT * st, sTf = sinT * ct + cosT * st;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) pos[k] = (pl.r[3 * k] * xf[0] + pl.r[3 * k + 1] * xf[1]) + pl.c[k];
+    const double d0 = cPf * sTf, d1 = sPf * sTf, d2 = cTf;
+    const double idn = 1.0 / sqrt(d0 * d0 + d1 * d1 + d2 * d2);
+    dir[0] = d0 * idn;
+    dir[1] = d1 * idn;
+    dir[2] = d2 * idn;
+    moved();
+    p = fabs(1.0 / xf[4]);
+    t = xf[5];
   }
 
   // ----------------------------------------------------------- navigation ----
@@ -973,7 +1099,7 @@ struct TrackFit {
       for (int m = 25; m < KF_SS_CF - KF_SS_J; ++m) jv[m] = (S)0;
       storeS<KF_SS_CF - KF_SS_J>(k, KF_SS_J, jv);
     }
-    if (tsf) {
+    if (tsf && lead) {
       storeBoundF(k, KF_TS_PREDICTED, xp, cov);
 #pragma unroll
       for (int r = 0; r < 6; ++r)
@@ -1060,7 +1186,7 @@ struct TrackFit {
     }
     correctParams(xf);
     storePar(k, KF_SD_XF, xf);
-    if (tsf) {
+    if (tsf && lead) {
       storeBoundF(k, KF_TS_FILTERED, xf, cov);
       *recF(k, KF_TS_CHI2) = chi2;
     }
@@ -1070,7 +1196,7 @@ struct TrackFit {
     S cm[KF_SS_N - KF_SS_CF];
 #pragma unroll
     for (int m = 0; m < 21; ++m) cm[m] = cov[m];
-    updateFromBound(pl, xf);
+    updateFromFiltered(pl, xf, xp);
     material(pl);
     cm[KF_SS_MD - KF_SS_CF + 0] = cov[sidx(2, 2)];
     cm[KF_SS_MD - KF_SS_CF + 1] = cov[sidx(3, 3)];
@@ -1095,21 +1221,21 @@ struct TrackFit {
     double sc[6];
 #pragma unroll
     for (int k = 0; k < 6; ++k) sc[k] = 1.0 / sqrt(C[sidx(k, k)]);
-    // W[r][k] = L[r][k] D[k] (r > k), L below the diagonal, Dinv = 1 / D
-    double L[6][6], W[6][6], Dinv[6];
+    // W[r][k] = Lo[r][k] D[k] (r > k), Lo below the diagonal, Dinv = 1 / D
+    double Lo[6][6], W[6][6], Dinv[6];
 #pragma unroll
     for (int j = 0; j < 6; ++j) {
       double dj = (sc[j] * C[sidx(j, j)]) * sc[j];
 #pragma unroll
-      for (int k = 0; k < j; ++k) dj = dj - W[j][k] * L[j][k];
+      for (int k = 0; k < j; ++k) dj = dj - W[j][k] * Lo[j][k];
       Dinv[j] = 1.0 / dj;
 #pragma unroll
       for (int r = j + 1; r < 6; ++r) {
         double w = (sc[r] * C[sidx(r, j)]) * sc[j];
 #pragma unroll
-        for (int k = 0; k < j; ++k) w = w - W[r][k] * L[j][k];
+        for (int k = 0; k < j; ++k) w = w - W[r][k] * Lo[j][k];
         W[r][j] = w;
-        L[r][j] = w * Dinv[j];
+        Lo[r][j] = w * Dinv[j];
       }
     }
 #pragma unroll
@@ -1119,7 +1245,7 @@ struct TrackFit {
       for (int r = 0; r < 6; ++r) {
         double v = sc[r] * M[r][c];
 #pragma unroll
-        for (int k = 0; k < r; ++k) v = v - L[r][k] * y[k];
+        for (int k = 0; k < r; ++k) v = v - Lo[r][k] * y[k];
         y[r] = v;
       }
 #pragma unroll
@@ -1128,7 +1254,7 @@ struct TrackFit {
       for (int r = 5; r >= 0; --r) {
         double v = y[r];
 #pragma unroll
-        for (int k = r + 1; k < 6; ++k) v = v - L[k][r] * y[k];
+        for (int k = r + 1; k < 6; ++k) v = v - Lo[k][r] * y[k];
         y[r] = v;
       }
 #pragma unroll
@@ -1362,6 +1488,7 @@ struct TrackFit {
   KFD_INLINE double &cd(int f) const { return a.carry_d[(size_t)f * a.count + cpos]; }
 
   KFD_INLINE void saveCarry(int resume, int steps, bool terminated) const {
+    if (!lead) return;
     ci(kCI_resume) = resume;
     ci(kCI_steps) = steps;
     ci(kCI_measured) = measured;
@@ -1430,6 +1557,7 @@ struct TrackFit {
   }
 
   KFD_INLINE void writeFinal(uint32_t status) const {
+    if (!lead) return;
     if (a.out_params)
 #pragma unroll
       for (int k = 0; k < 6; ++k) a.out_params[(size_t)k * a.n + i] = fpar[k];
@@ -1497,13 +1625,15 @@ struct TrackFit {
         }
       }
     }
-    if (a.out_chi2) a.out_chi2[i] = chi2sum;
-    if (a.out_ndf) a.out_ndf[i] = 2 * (measured - outliers);
-    if (a.out_nmeas) a.out_nmeas[i] = measured;
+    if (lead) {
+      if (a.out_chi2) a.out_chi2[i] = chi2sum;
+      if (a.out_ndf) a.out_ndf[i] = 2 * (measured - outliers);
+      if (a.out_nmeas) a.out_nmeas[i] = measured;
+    }
     if (resume != kResumeNone) {
       saveCarry(resume, steps, terminated);
     } else {
-      ci(kCI_resume) = kResumeNone;
+      if (lead) ci(kCI_resume) = kResumeNone;
       writeFinal(bad ? KF_STATUS_BAD_INPUT : statusBits(steps, terminated));
     }
   }

@@ -175,7 +175,9 @@ typedef struct kf_schedule {
   int32_t persistent;      /* 1: resident waves pull 64-track chunks from a work queue */
   int32_t waves_per_cu;    /* persistent grid size, waves per compute unit (default 4) */
   int32_t chunks_per_grab; /* work-queue granularity, 64-track chunks per grab (default 1) */
-  int32_t lanes_per_track; /* lanes cooperating on one track: 1 (the only variant) */
+  int32_t lanes_per_track; /* lanes cooperating on one track (the counterpart of the
+                              reference's block-per-track kernel, Kernels.cu:47-94):
+                              1 (default), 2 or 4 consecutive lanes, one-shot grid only */
   int32_t graph;           /* 1: capture the launch sequence as a hipGraph on the first
                               call and replay it while the arguments are unchanged */
 } kf_schedule;
@@ -267,8 +269,8 @@ int kf_get_schedule(const kf_ctx *ctx, kf_schedule *s);
  * one-shot grid or persistent work queue with 2/4/8 waves per CU and 1/4
  * chunks per grab, graph replay), times every candidate on `stream` with HIP
  * events (best of `repeats` fits), keeps the fastest as ctx's schedule and
- * returns it with its time.  The results buffers hold the last fit.
- * lanes_per_track is 1 in every candidate. */
+ * returns it with its time.  The results buffers hold the last fit.  The
+ * one-shot candidates also run with 2 and 4 lanes per track. */
 int kf_autotune(kf_ctx *ctx, const kf_tracks *tracks, const kf_options *opt,
                 const kf_results *res, void *stream, int32_t repeats, kf_schedule *best,
                 double *best_ms);

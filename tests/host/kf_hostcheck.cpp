@@ -36,8 +36,12 @@ struct Host {
   std::vector<uint32_t> out_status;
   size_t per = 0;
 
+  std::vector<double> tsf;
+  std::vector<float> ssf;
+
   Host(const kf_plane *pl, int32_t np, const kfh_field *f, const kf_tracks *tr,
-       const kf_options *opt) {
+       const kf_options *opt, bool smoothing = false, bool track_states = false,
+       bool mixed = false) {
     std::memset(&a, 0, sizeof a);
     for (int32_t p = 0; p < np; ++p) planes.push_back(devPlane(pl[p]));
     const int64_t n = tr->n_tracks;
@@ -67,7 +71,15 @@ struct Host {
       a.field.iw[k] = f->grid ? 1.0 / a.field.w[k] : 0.0;
     }
     a.field.values = f->values;
-    a.smoothing = 0;
+    a.smoothing = smoothing ? opt->smoothing : 0;
+    a.outlier_chi2_cut = opt->outlier_chi2_cut;
+    a.unique_geo = 1;
+    for (int32_t p = 0; p < np && a.unique_geo; ++p)
+      for (int32_t q = p + 1; q < np; ++q)
+        if (pl[p].geo_id == pl[q].geo_id) {
+          a.unique_geo = 0;
+          break;
+        }
     a.ms = opt->multiple_scattering;
     a.eloss = opt->energy_loss;
     a.max_steps = opt->max_steps;
@@ -99,8 +111,17 @@ struct Host {
     ck.resize(chunks * per * KF_CK_N);
     carry_i.resize(6 * n);
     carry_d.resize(30 * n);
+    if (mixed) {
+      ssf.resize(chunks * per * KF_SS_N);
+      a.tss = ssf.data();
+    } else {
+      a.tss = ss.data();
+    }
+    if (track_states) {
+      tsf.resize(chunks * per * KF_TS_STRIDE);
+      a.tsf = tsf.data();
+    }
     a.tsd = sd.data();
-    a.tss = ss.data();
     a.tsc = ck.data();
     a.carry_i = carry_i.data();
     a.carry_d = carry_d.data();
@@ -115,11 +136,65 @@ struct Host {
       tf.runForward();
     }
   }
+
+  // The whole fit, both kernels' programs one track at a time.
+  template <typename S>
+  void fit() {
+    a.tsc = nullptr;
+    for (int pass = 0; pass < 2; ++pass)
+      for (int64_t i = 0; i < a.n; ++i) {
+        const size_t ch = (size_t)(i >> 6);
+        double *d = a.tsd + ch * per * KF_SD_N;
+        S *s = (S *)a.tss + ch * per * KF_SS_N;
+        double *f = a.tsf ? a.tsf + ch * per * KF_TS_STRIDE : nullptr;
+        if (pass == 0) {
+          TrackFit<S, true> tf(a, i, d, s, f, nullptr, (uint32_t)(i & 63), i);
+          tf.pls = planes.data();
+          tf.runForward();
+        } else {
+          TrackFit<S, false> tf(a, i, d, s, f, nullptr, (uint32_t)(i & 63), i);
+          tf.pls = planes.data();
+          tf.runBackward();
+        }
+      }
+  }
 };
 
 }  // namespace
 
 extern "C" {
+
+// The fit of every track by the kernels' per-track program (FP64 or mixed):
+// params [6][n], cov [21][n], chi2, ndf, n_measured, status [n], and, when
+// track_states is not NULL, the track-state records [max_states][KF_TS_STRIDE][n].
+int kfh_fit(const kf_plane *planes, int32_t n_planes, const kfh_field *f, const kf_tracks *tr,
+            const kf_options *opt, double *params, double *cov, double *chi2, int32_t *ndf,
+            int32_t *n_measured, uint32_t *status, double *track_states) {
+  const bool mixed = opt->precision == KF_PRECISION_MIXED;
+  Host h(planes, n_planes, f, tr, opt, true, track_states != nullptr, mixed);
+  if (mixed)
+    h.fit<float>();
+  else
+    h.fit<double>();
+  const int64_t n = tr->n_tracks;
+  std::memcpy(params, h.out_params.data(), 6 * n * sizeof(double));
+  std::memcpy(cov, h.out_cov.data(), 21 * n * sizeof(double));
+  std::memcpy(chi2, h.out_chi2.data(), n * sizeof(double));
+  std::memcpy(ndf, h.out_ndf.data(), n * sizeof(int32_t));
+  std::memcpy(n_measured, h.out_nmeas.data(), n * sizeof(int32_t));
+  std::memcpy(status, h.out_status.data(), n * sizeof(uint32_t));
+  if (track_states) {
+    const int S = h.a.max_states;
+    for (int64_t i = 0; i < n; ++i) {
+      const double *base = h.tsf.data() + (size_t)(i >> 6) * h.per * KF_TS_STRIDE;
+      for (int k = 0; k < h.out_nmeas[i] && k < S; ++k)
+        for (int c = 0; c < KF_TS_STRIDE; ++c)
+          track_states[((size_t)k * KF_TS_STRIDE + c) * n + i] =
+              base[((size_t)k * KF_TS_STRIDE + c) * 64 + (i & 63)];
+    }
+  }
+  return 0;
+}
 
 // Summed-chi2 objective per track (forward filter, FP64, smoothing off).
 int kfh_chi2(const kf_plane *planes, int32_t n_planes, const kfh_field *f, const kf_tracks *tr,

@@ -31,7 +31,7 @@ def test_driver_presets_match_python_workloads(built, cfg):
         assert got[k] == pytest.approx(v, rel=0, abs=0), k
 
 
-@pytest.mark.parametrize("args", [["-d", "cpu"], ["-u", "1"], ["-s", "1"], ["-c", "4"],
+@pytest.mark.parametrize("args", [["-d", "cpu"], ["-u", "1"], ["-c", "4"],
                                   ["-p", "fp16"], ["--bogus"]])
 def test_driver_refuses_what_this_build_does_not_do(built, args):
     p = run(args)
@@ -78,3 +78,23 @@ def test_driver_with_copies_gives_the_same_fit(built, tmp_path):
     assert "copied-out result elements differing from the device's: 0" in pb.stdout, pb.stdout
     name = f"params-fitted_config3_{n}.csv"
     assert (a / name).read_bytes() == (b / name).read_bytes()
+
+
+@pytest.mark.gpu
+def test_driver_lane_groups_fit_the_same_tracks(built, tmp_path):
+    """kf_run -s 1 (the reference's block-per-track switch) fits every track
+    on a group of 4 lanes: the same statuses and CSV rows as one lane per
+    track, parameters within 1e-12 of them."""
+    n = 5000
+    a, b = tmp_path / "a", tmp_path / "b"
+    a.mkdir()
+    b.mkdir()
+    pa = run(["-c", "3", "-t", str(n), "-o", "1", "-n", "2"], cwd=a)
+    pb = run(["-c", "3", "-t", str(n), "-o", "1", "-n", "2", "-s", "1"], cwd=b)
+    assert pa.returncode == 0 and pb.returncode == 0, pa.stderr + pb.stderr
+    assert "4 lane(s) per track" in pb.stdout, pb.stdout
+    name = f"params-fitted_config3_{n}.csv"
+    ca = np.loadtxt(a / name, delimiter=",", skiprows=1)
+    cb = np.loadtxt(b / name, delimiter=",", skiprows=1)
+    assert np.array_equal(ca[:, 0], cb[:, 0])
+    assert np.abs(ca[:, 1:] - cb[:, 1:]).max() <= 1e-12 * np.abs(ca[:, 1:]).max()

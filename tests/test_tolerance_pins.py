@@ -11,6 +11,9 @@ the computation, not a defect of the kernels.
   filtered covariances and Jacobians in float; rounding the RTS operands to
   float moves the oracle's smoothed covariances by up to ~1e-4, a 1500-fold
   amplification of float's 6e-8.
+* Track-state covariances on the material workload (bar 1e-6): one ulp of
+  the measurements moves the oracle's own filtered and smoothed covariances
+  there by up to 8e-8.
 * Rotations in the adjoint's finite differences (step 1e-8 rad, not 1e-6):
   a 1e-6 rad tilt changes the number of RK steps of some tracks (the
   navigator's 1e-4 mm on-surface tolerance) and moves the central difference
@@ -26,7 +29,7 @@ from gpukalmanfitter_amd import generate as G
 from gpukalmanfitter_amd.fitter import unpack_sym
 
 import oracle
-from parity import cov_dev
+from parity import compare_states, cov_dev
 
 W = configs.WORKLOAD_FIT_OPTIONS
 
@@ -44,6 +47,30 @@ def test_ms_off_perigee_covariance_moves_1e9_under_one_ulp(built):
     print("perigee covariance moved by one ulp of the start covariance:", dev)
     assert dev[1] < 5e-10          # with scattering: inside the 1e-9 bar
     assert 1e-9 < dev[0] < 1e-8    # without: beyond 1e-9, inside the 1e-8 bar
+
+
+def test_material_state_covariances_move_4e8_under_one_ulp(built):
+    """On the material workload, one ulp of each measurement moves the
+    oracle's own filtered and smoothed state covariances by more than 1e-8
+    (low-momentum tracks grazing a plane late in the track: updates that
+    shrink a variance by up to 1e6), where the no-material ragged workload
+    stays below 1e-9: the FP64 bar of those state covariances is 1e-6."""
+    rng = np.random.default_rng(3)
+    dev = {}
+    for cfg in (3, 5):
+        b = G.generate(cfg, n_tracks=2000)
+        opt = abi.default_options(**W)
+        o1, _, _, _ = oracle.fit(b, opt, track_states=True)
+        flip = rng.choice([-1.0, 1.0], b.hit_meas.shape)
+        b2 = dataclasses.replace(b, hit_meas=b.hit_meas * (1 + 2.0 ** -52 * flip))
+        o2, _, _, _ = oracle.fit(b2, opt, track_states=True)
+        sm = ((o1.status & abi.KF_STATUS_SMOOTHED) != 0) & \
+            ((o1.status & abi.KF_STATUS_SMOOTHER_NAN) == 0)
+        w = compare_states(o2.track_states, o1.track_states, o1.n_measured, sm)
+        dev[cfg] = max(w["filtered_cov"], w["smoothed_cov"])
+    print("state covariances moved by one ulp of the measurements:", dev)
+    assert dev[3] < 1e-9
+    assert 1e-8 < dev[5] < 1e-6
 
 
 def test_float_rts_operands_move_smoothed_covariance(built):
