@@ -154,11 +154,19 @@ struct TrackFit {
   }
   KFD_INLINE double *grpD(int k, int f) const { return tsd + gslot(k, KF_SD_N, f, 2, lane); }
   KFD_INLINE S *grpS(int k, int f) const { return tss + gslot(k, KF_SS_N, f, kW, lane); }
+  // The forward kernel's store writes are non-temporal: the backward kernel
+  // reads them back only after the whole forward launch (gigabytes later),
+  // so they need not displace the hits and plane data in the caches
+  // (measured: forward 2-3 %, backward 3 % faster on 1,048,576 tracks).
+  typedef float v4f __attribute__((ext_vector_type(4)));
+  typedef double v2d __attribute__((ext_vector_type(2)));
   KFD_INLINE static void st16(float *p, const float *v) {
-    *(float4 *)p = make_float4(v[0], v[1], v[2], v[3]);
+    const v4f x = {v[0], v[1], v[2], v[3]};
+    __builtin_nontemporal_store(x, (v4f *)p);
   }
   KFD_INLINE static void st16(double *p, const double *v) {
-    *(double2 *)p = make_double2(v[0], v[1]);
+    const v2d x = {v[0], v[1]};
+    __builtin_nontemporal_store(x, (v2d *)p);
   }
   KFD_INLINE static void ld16(const float *p, float *v) {
     const float4 x = *(const float4 *)p;
