@@ -34,6 +34,9 @@ def _lib(counting=False):
     if name in _LIBS:
         return _LIBS[name]
     path = os.path.join(_HERE, name)
+    if not counting and os.environ.get("KFO_LIBRARY"):
+        # another build of the same sources (the sanitizer build)
+        path = os.environ["KFO_LIBRARY"]
     if not os.path.exists(path):
         build()
     lib = C.CDLL(path)
