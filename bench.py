@@ -64,8 +64,8 @@ HBM_PEAK_GBS = 8000.0
 # calibration (profiles/kf_fit_config2_1M_tracks.md).
 TRAFFIC_TRACKS = 1048576
 TRAFFIC_BYTES_PER_LAUNCH = {
-    ("fp64", "forward"): 4.747e9 + 5.662e9, ("fp64", "backward"): 5.343e9 + 0.231e9,
-    ("mixed", "forward"): 4.502e9 + 3.481e9, ("mixed", "backward"): 3.344e9 + 0.231e9,
+    ("fp64", "forward"): 2.276e9 + 5.663e9, ("fp64", "backward"): 5.333e9 + 0.231e9,
+    ("mixed", "forward"): 2.152e9 + 3.481e9, ("mixed", "backward"): 3.344e9 + 0.231e9,
 }
 PARAM_NAMES = ("loc0", "loc1", "phi", "theta", "qop", "t")
 
@@ -93,6 +93,12 @@ def parse():
                     help="skip the 10,000-track config-1 leg")
     ap.add_argument("--no-adjoint", action="store_true",
                     help="skip the chi2-adjoint leg (config 4, 262,144 tracks)")
+    ap.add_argument("--field", choices=("grid", "flat", "const"), default="grid",
+                    help="main loop on a grid-field workload (--config 3 or 5): its grid "
+                         "(default), a grid of the same box holding the grid's central "
+                         "value everywhere (same trajectories as a constant field: the "
+                         "difference to 'const' is the cost of the lookups), or that "
+                         "constant field")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only copy a MIB-sized device buffer (calibrates rocprofv3 "
                          "FETCH_SIZE / WRITE_SIZE against known bytes) and exit")
@@ -593,6 +599,16 @@ def main():
     gen_s = time.time() - t0
     n = batch.n_tracks
     print(f"[bench] generated {n} tracks ({cfg['name']}) in {gen_s:.1f}s", file=sys.stderr, flush=True)
+    if args.field != "grid" and batch.field.kind == "grid":
+        v = batch.field.values
+        c = v[v.shape[0] // 2, v.shape[1] // 2, v.shape[2] // 2]
+        if args.field == "flat":
+            batch.field = G.FieldSpec(kind="grid", values=np.broadcast_to(c, v.shape).copy(),
+                                      lo=batch.field.lo, hi=batch.field.hi)
+        else:
+            batch.field = G.FieldSpec(kind="const", b=tuple(float(x) for x in c))
+        print(f"[bench] field: {args.field} ({tuple(float(x) for x in c)})", file=sys.stderr,
+              flush=True)
 
     kf = KalmanFitter(batch.planes, batch.field)
     t0 = time.time()
@@ -713,7 +729,8 @@ def main():
         "config": {"workload": cfg["name"] + f": {n} tracks, " + (
             "10 x-normal planes x=30..300 mm, Bz=2 T, no material, mu+- p 1-10 GeV" if args.config in (1, 2)
             else "see gpukalmanfitter_amd/configs.py"), "n_tracks": n, "precision": head,
-            "overstep_limit_mm": configs.WORKLOAD_FIT_OPTIONS["overstep_limit"]},
+            "overstep_limit_mm": configs.WORKLOAD_FIT_OPTIONS["overstep_limit"],
+            **({"field": args.field} if args.field != "grid" else {})},
         "tracks_per_s": rates,
         "pulls": pulls,
         "end_to_end": e2e,

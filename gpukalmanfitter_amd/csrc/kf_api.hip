@@ -4,12 +4,14 @@
 // per-state store used by the smoother (grown on demand, never shrunk), and
 // two HIP events that time the last fit kernel on its launch stream.
 #include <hip/hip_runtime.h>
+#include <rocprofiler-sdk-roctx/roctx.h>
 
 #include <algorithm>
 #include <cfloat>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <new>
@@ -65,6 +67,29 @@ struct kf_ctx {
 namespace {
 
 char g_create_err[512] = {0};
+
+// Host-side profiler ranges around the entry points (the reference's
+// PUSH_RANGE / POP_RANGE around KalmanFitter::fit, Utilities/Profiling.hpp:
+// 12-26, KalmanFitter.hpp:728,758), shown by rocprofv3 --marker-trace; on
+// when the environment holds KF_ROCTX=1 when the library first asks.
+bool rangesOn() {
+  static const bool on = [] {
+    const char *v = std::getenv("KF_ROCTX");
+    return v && v[0] == '1' && v[1] == 0;
+  }();
+  return on;
+}
+struct Range {
+  bool on;
+  explicit Range(const char *name) : on(rangesOn()) {
+    if (on) roctxRangePushA(name);
+  }
+  ~Range() {
+    if (on) roctxRangePop();
+  }
+  Range(const Range &) = delete;
+  Range &operator=(const Range &) = delete;
+};
 
 // Device addresses of `scratch`, per device (resolved with that device
 // current); g_ws_state: 0 unresolved, 1 resolved, < 0 the hipError_t of a
@@ -466,6 +491,7 @@ extern "C" {
 
 int kf_fit_noalloc(kf_ctx *ctx, const kf_tracks *tr, const kf_options *opt,
                    const kf_results *res, void *stream) {
+  const Range range("kf_fit_noalloc");
   kfd::FitArgs a;
   int rc = prepareFit(ctx, tr, opt, res, "kf_fit_noalloc", a);
   if (rc != KF_OK || tr->n_tracks == 0) return rc;
@@ -518,6 +544,7 @@ int kf_fit_noalloc(kf_ctx *ctx, const kf_tracks *tr, const kf_options *opt,
 
 int kf_fit(kf_ctx *ctx, const kf_tracks *tr, const kf_options *opt, const kf_results *res,
            void *stream) {
+  const Range range("kf_fit");
   kfd::FitArgs a;
   int rc = prepareFit(ctx, tr, opt, res, "kf_fit", a);
   if (rc != KF_OK || tr->n_tracks == 0) return rc;
@@ -566,6 +593,7 @@ int kf_fit(kf_ctx *ctx, const kf_tracks *tr, const kf_options *opt, const kf_res
       ctx->gexec = nullptr;
       ctx->graph = nullptr;
       ctx->gkey.clear();
+      const Range capture("kf_fit: graph capture");
       KF_HIP(ctx, hipStreamSynchronize(s));
       KF_HIP(ctx, hipStreamBeginCapture(ctx->cap_stream, hipStreamCaptureModeThreadLocal));
       e = launchSlices(a, L, opt->precision, res->track_states, slice, nullptr, ctx->cap_stream);
@@ -628,6 +656,7 @@ int kf_get_schedule(const kf_ctx *ctx, kf_schedule *s) {
 
 int kf_autotune(kf_ctx *ctx, const kf_tracks *tracks, const kf_options *opt, const kf_results *res,
                 void *stream, int32_t repeats, kf_schedule *best, double *best_ms) {
+  const Range range("kf_autotune");
   if (!ctx) return fail(nullptr, KF_E_INVALID, "kf_autotune: ctx is NULL");
   if (!tracks || !opt || !res) return fail(ctx, KF_E_INVALID, "kf_autotune: NULL argument");
   if (repeats < 1) repeats = 1;
@@ -683,6 +712,7 @@ int kf_autotune(kf_ctx *ctx, const kf_tracks *tracks, const kf_options *opt, con
 int kf_chi2_gradient(kf_ctx *ctx, const kf_tracks *tr, const kf_options *opt, double *grad_planes,
                      double *grad_field_scale, double *chi2_sum, double *grad_raw,
                      uint32_t *track_flags, void *stream) {
+  const Range range("kf_chi2_gradient");
   if (!ctx) return fail(nullptr, KF_E_INVALID, "kf_chi2_gradient: ctx is NULL");
   if (!tr || !opt || !grad_planes || !grad_field_scale)
     return fail(ctx, KF_E_INVALID, "kf_chi2_gradient: NULL tracks/options/outputs");
@@ -784,6 +814,7 @@ int64_t kf_intersect_scratch_bytes(int64_t n_rays, int64_t n_surfaces) {
 int kf_intersect(const kf_polygon_plane *surfaces, int64_t n_surfaces, const double *ray_pos,
                  const double *ray_dir, int64_t n_rays, int32_t bcheck, kf_intersection *hits,
                  int64_t capacity, int64_t *n_hits, void *work, void *stream) {
+  const Range range("kf_intersect");
   if (n_surfaces < 0 || n_rays < 0 || capacity < 0)
     return fail(nullptr, KF_E_INVALID, "kf_intersect: negative size");
   if (n_surfaces > INT32_MAX || n_rays > INT32_MAX)
