@@ -54,10 +54,10 @@ FP32_PEAK_TFLOPS = 157.3
 EXT_BYTES_PER_TRACK = {"forward": 712, "backward": 268}
 # The same plus the launch's working storage (tests/test_bench_constants.py
 # re-derives them from the layouts): the smoother store, 10 measured states x
-# (12 x 8 + 49 x sizeof(S)), written forward and read backward, and the
-# 264-byte forward-to-backward hand-over (6 int32 + 30 double).
-DESIGN_BYTES_PER_TRACK = {("fp64", "forward"): 5856, ("fp64", "backward"): 5412,
-                          ("mixed", "forward"): 3896, ("mixed", "backward"): 3452}
+# (12 x 8 + 50 x sizeof(S)), written forward and read backward, and the
+# 272-byte forward-to-backward hand-over (8 int32 + 30 double).
+DESIGN_BYTES_PER_TRACK = {("fp64", "forward"): 5944, ("fp64", "backward"): 5500,
+                          ("mixed", "forward"): 3944, ("mixed", "backward"): 3500}
 HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch (fetched + written) of the two fit kernels on this
 # workload, from rocprofv3 FETCH_SIZE / WRITE_SIZE scaled by the copy-kernel

@@ -372,7 +372,7 @@ int setupArgs(kf_ctx *ctx, const kf_tracks *tr, const kf_options *opt, const cha
   return KF_OK;
 }
 
-constexpr size_t kCarryBytes = 6 * sizeof(int32_t) + 30 * sizeof(double);
+constexpr size_t kCarryBytes = KF_CARRY_I * sizeof(int32_t) + KF_CARRY_D * sizeof(double);
 
 // Working-buffer bytes per track of the fit: per-state stores and hand-over.
 size_t storeBytesPerTrack(const kfd::FitArgs &a, int precision, bool track_states,
@@ -444,7 +444,7 @@ int setupStores(kf_ctx *ctx, int precision, bool track_states, bool checkpoints,
 void sliceRange(kfd::FitArgs &a, int64_t first, int64_t count) {
   a.first = first;
   a.count = count;
-  a.carry_i = (int32_t *)(a.carry_d + 30 * (size_t)count);
+  a.carry_i = (int32_t *)(a.carry_d + KF_CARRY_D * (size_t)count);
 }
 
 // The whole fit of a batch: the length sort over all positions, then the

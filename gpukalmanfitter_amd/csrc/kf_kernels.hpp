@@ -13,15 +13,23 @@
 // count); larger tables are read from global memory.
 #define KF_LDS_PLANES 256
 
-// per-state store elements (see FitArgs)
+// per-state store elements (see FitArgs).  A measured state sits in the
+// store slot of its plane's position in the track's surface sequence (when
+// the sequence fits max_states slots; else in slot = its measurement index),
+// so the lanes of a wave that cross the same plane write and read the same
+// 64-lane chunk of the store even when their tracks have holes.
 #define KF_SD_XP 0   // predicted parameters (6)
 #define KF_SD_XF 6   // filtered parameters (6)
 #define KF_SD_N 12
 #define KF_SS_J 0    // Jacobian rows 0, 1, 2, 3, 5 x columns 0..4 (25, padded to 28)
+#define KF_SS_LINK 25  // in the padding: slot of the measured state two before (bits)
 #define KF_SS_CF 28  // filtered covariance (21, packed upper triangle)
 #define KF_SS_MD 49  // covariance (2,2), (3,3), (4,4) after the material update
 #define KF_SS_N 52
-#define KF_SS_USED 49  // elements that carry data (the rest is padding)
+#define KF_SS_USED 50  // elements that carry data (the rest is padding)
+// forward -> backward hand-over record per track: int32 and double fields
+#define KF_CARRY_I 8
+#define KF_CARRY_D 30
 // per-surface checkpoints of the chi2 adjoint (state after the update and
 // material at a measured surface; see FitArgs::tsc)
 #define KF_CK_POS 0    // 3
@@ -99,8 +107,8 @@ struct FitArgs {
   //       (NULL outside the adjoint).
   double *tsc;
   // forward -> backward kernel hand-over, [field][n]
-  int32_t *carry_i;  // 6 int fields
-  double *carry_d;   // 30 double fields
+  int32_t *carry_i;  // KF_CARRY_I int fields
+  double *carry_d;   // KF_CARRY_D double fields
   // Scheduling.  Tracks are processed at positions 0..n-1; the track at
   // position p is order[p] (identity when NULL) and its store slot is that
   // of position p.  persistent = 1: a grid of resident waves pulls

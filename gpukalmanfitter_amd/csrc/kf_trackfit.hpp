@@ -74,6 +74,11 @@ struct TrackFit {
   int measured;
   int outliers;    // measured states the outlier finder left without an update
   int firstPlane;  // plane-table index of the first measured state
+  // store slots of the last two measured states (KF_SD_* / KF_SS_*: the slot
+  // of a state is its plane's position in the sequence when the sequence
+  // fits max_states slots, else its measurement index)
+  int slotLast, slotPrev;
+  KFD_INLINE bool slotByPlane() const { return size <= a.max_states; }
   bool smoothed, finished, result, smootherNaN;
   int64_t h0, h1;
   // hit search: while every match so far was the next hit in order and the
@@ -168,6 +173,13 @@ struct TrackFit {
     const v2d x = {v[0], v[1]};
     __builtin_nontemporal_store(x, (v2d *)p);
   }
+  // a store slot index kept in a padding element of the S store, bit for bit
+  KFD_INLINE static float linkBits(int s, float) { return __builtin_bit_cast(float, s); }
+  KFD_INLINE static double linkBits(int s, double) {
+    return __builtin_bit_cast(double, (int64_t)s);
+  }
+  KFD_INLINE static int linkOf(float v) { return __builtin_bit_cast(int, v); }
+  KFD_INLINE static int linkOf(double v) { return (int)__builtin_bit_cast(int64_t, v); }
   KFD_INLINE static void ld16(const float *p, float *v) {
     const float4 x = *(const float4 *)p;
     v[0] = x.x;
@@ -1090,12 +1102,14 @@ struct TrackFit {
     }
     if (measured >= a.max_states) return false;
     const int k = measured;
+    const int s = slotByPlane() ? sp - seqFirst : k;  // store slot
     if (k == 0) firstPlane = sp;
     double xp[6];
     S jb[6][6];
     boundStatePlane(pl, xp, jb);
-    // predicted parameters and the transport Jacobian to the store
-    storePar(k, KF_SD_XP, xp);
+    // predicted parameters and the transport Jacobian to the store, with the
+    // slot of the state two measurements back (the smoother's look-ahead)
+    storePar(s, KF_SD_XP, xp);
     {
       S jv[KF_SS_CF - KF_SS_J];
 #pragma unroll
@@ -1103,9 +1117,10 @@ struct TrackFit {
 #pragma unroll
         for (int c = 0; c < 5; ++c)
           if (r != 4) jv[jidx(r, c)] = jb[r][c];
+      jv[KF_SS_LINK] = linkBits(slotPrev, (S)0);
 #pragma unroll
-      for (int m = 25; m < KF_SS_CF - KF_SS_J; ++m) jv[m] = (S)0;
-      storeS<KF_SS_CF - KF_SS_J>(k, KF_SS_J, jv);
+      for (int m = KF_SS_LINK + 1; m < KF_SS_CF - KF_SS_J; ++m) jv[m] = (S)0;
+      storeS<KF_SS_CF - KF_SS_J>(s, KF_SS_J, jv);
     }
     if (tsf && lead) {
       storeBoundF(k, KF_TS_PREDICTED, xp, cov);
@@ -1193,7 +1208,7 @@ struct TrackFit {
     chi2sum = chi2sum + chi2;
     }
     correctParams(xf);
-    storePar(k, KF_SD_XF, xf);
+    storePar(s, KF_SD_XF, xf);
     if (tsf && lead) {
       storeBoundF(k, KF_TS_FILTERED, xf, cov);
       *recF(k, KF_TS_CHI2) = chi2;
@@ -1209,8 +1224,10 @@ struct TrackFit {
     cm[KF_SS_MD - KF_SS_CF + 0] = cov[sidx(2, 2)];
     cm[KF_SS_MD - KF_SS_CF + 1] = cov[sidx(3, 3)];
     cm[KF_SS_MD - KF_SS_CF + 2] = cov[sidx(4, 4)];
-    storeS<KF_SS_N - KF_SS_CF>(k, KF_SS_CF, cm);
+    storeS<KF_SS_N - KF_SS_CF>(s, KF_SS_CF, cm);
     if (tsc) saveCheckpoint(k);
+    slotPrev = slotLast;
+    slotLast = s;
     ++measured;
     return true;
   }
@@ -1278,46 +1295,60 @@ struct TrackFit {
     for (int m = 0; m < 21; ++m) c[m] = (double)cm[m];
   }
 
-  // Stored operands of smoother step k: the Jacobian and predicted
-  // parameters of state k + 1, the filtered covariance, post-material
-  // diagonal and filtered parameters of state k.
+  // Stored operands of smoother step k: the Jacobian, predicted parameters
+  // and link of state k + 1 (slot sn), the filtered covariance,
+  // post-material diagonal and filtered parameters of state k (slot sk).
   struct SmoothIn {
     S j[25];
     S cf[21];
     S md[3];
     double xf[6];
     double xpn[6];
+    int link;  // slot of state k - 1 (-1: none)
   };
-  KFD_INLINE void loadSmoothIn(int k, SmoothIn &in) const {
+  KFD_INLINE void loadSmoothIn(int sk, int sn, SmoothIn &in) const {
     S jv[KF_SS_CF - KF_SS_J], cm[KF_SS_N - KF_SS_CF];
-    loadS<KF_SS_CF - KF_SS_J>(k + 1, KF_SS_J, jv);
-    loadS<KF_SS_N - KF_SS_CF>(k, KF_SS_CF, cm);
+    loadS<KF_SS_CF - KF_SS_J>(sn, KF_SS_J, jv);
+    loadS<KF_SS_N - KF_SS_CF>(sk, KF_SS_CF, cm);
 #pragma unroll
     for (int m = 0; m < 25; ++m) in.j[m] = jv[m];
+    in.link = linkOf(jv[KF_SS_LINK]);
 #pragma unroll
     for (int m = 0; m < 21; ++m) in.cf[m] = cm[m];
 #pragma unroll
     for (int m = 0; m < 3; ++m) in.md[m] = cm[KF_SS_MD - KF_SS_CF + m];
-    loadPar(k, KF_SD_XF, in.xf);
-    loadPar(k + 1, KF_SD_XP, in.xpn);
+    loadPar(sk, KF_SD_XF, in.xf);
+    loadPar(sn, KF_SD_XP, in.xpn);
   }
 
   // GainMatrixSmoother (GainMatrixSmoother.hpp:21-94):
   // G = (C_f J^T) C_p^-1, x_s = x_f + G (x_s' - x_p'), C_s = C_f - G (C_p - C_s') G^T,
   // in double in both modes.  The operands of step k - 1 are loaded while
-  // step k computes.
+  // step k computes.  The loop walks the store slots downwards from the last
+  // one the sequence can use and smooths a state when it reaches its slot,
+  // so the lanes of a wave whose tracks cross the same planes read the same
+  // store chunk in the same iteration, holes or not (a lane idles at the
+  // slots of its holes); it ends after at most max_states iterations.
   KFD_INLINE bool smooth(double xs[6], double cs[21]) {
     const int n = measured;
     double xsn[6], csn[21];
     // last state: smoothed = filtered
-    loadPar(n - 1, KF_SD_XF, xsn);
-    loadFilteredCov(n - 1, csn);
+    loadPar(slotLast, KF_SD_XF, xsn);
+    loadFilteredCov(slotLast, csn);
     if (tsf) storeBoundF(n - 1, KF_TS_SMOOTHED, xsn, csn);
     SmoothIn cur;
-    if (n >= 2) loadSmoothIn(n - 2, cur);
-    for (int k = n - 2; k >= 0; --k) {
+    int sk = slotPrev;  // slot of state k
+    if (n >= 2) loadSmoothIn(sk, slotLast, cur);
+    int slot = slotByPlane() ? size - 1 : sk;
+    for (int k = n - 2; k >= 0; --slot) {
+      if (slot < 0 || sk < 0) {  // slots out of order: the store is inconsistent
+        smootherNaN = true;
+        return false;
+      }
+      if (slot != sk) continue;  // a hole of this track
+      const int link = cur.link;
       SmoothIn nxt;
-      loadSmoothIn((k > 0) ? k - 1 : 0, nxt);
+      loadSmoothIn((k > 0 && link >= 0 && link < sk) ? link : sk, sk, nxt);
       S J[6][6];
 #pragma unroll
       for (int r = 0; r < 6; ++r)
@@ -1408,6 +1439,8 @@ struct TrackFit {
       correctParams(xsn);
       if (tsf) storeBoundF(k, KF_TS_SMOOTHED, xsn, csn);
       cur = nxt;
+      sk = (k > 0) ? link : 0;
+      --k;
     }
 #pragma unroll
     for (int m = 0; m < 6; ++m) xs[m] = xsn[m];
@@ -1490,8 +1523,12 @@ struct TrackFit {
 
   // ------------------------------------------------------------- carry ----
   // Per-track hand-over record from the forward to the backward kernel.
-  enum { kCI_resume = 0, kCI_steps, kCI_measured, kCI_first, kCI_next, kCI_flags, kCI_N };
+  enum {
+    kCI_resume = 0, kCI_steps, kCI_measured, kCI_first, kCI_next, kCI_flags, kCI_slotLast,
+    kCI_slotPrev, kCI_N
+  };
   enum { kCD_q = 0, kCD_J = 1, kCD_k4 = 26, kCD_dtds = 29, kCD_N = 30 };
+  static_assert(kCI_N == KF_CARRY_I && kCD_N == KF_CARRY_D, "hand-over record layout");
   KFD_INLINE int32_t &ci(int f) const { return a.carry_i[(size_t)f * a.count + cpos]; }
   KFD_INLINE double &cd(int f) const { return a.carry_d[(size_t)f * a.count + cpos]; }
 
@@ -1502,6 +1539,8 @@ struct TrackFit {
     ci(kCI_measured) = measured;
     ci(kCI_first) = firstPlane;
     ci(kCI_next) = next;
+    ci(kCI_slotLast) = slotLast;
+    ci(kCI_slotPrev) = slotPrev;
     ci(kCI_flags) = (result ? 1 : 0) | (stepFailed ? 2 : 0) | (navBreak ? 4 : 0) |
                     (terminated ? 8 : 0) | (derivSet ? 16 : 0);
     cd(kCD_q) = q;
@@ -1525,6 +1564,8 @@ struct TrackFit {
     measured = ci(kCI_measured);
     firstPlane = ci(kCI_first);
     next = ci(kCI_next);
+    slotLast = ci(kCI_slotLast);
+    slotPrev = ci(kCI_slotPrev);
     const int fl = ci(kCI_flags);
     result = (fl & 1) != 0;
     stepFailed = (fl & 2) != 0;
@@ -1606,6 +1647,7 @@ struct TrackFit {
     measured = 0;
     outliers = 0;
     firstPlane = 0;
+    slotLast = slotPrev = -1;
     result = true;
     chi2sum = 0.0;
     next = 0;

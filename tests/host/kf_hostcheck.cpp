@@ -109,8 +109,8 @@ struct Host {
     sd.resize(chunks * per * KF_SD_N);
     ss.resize(chunks * per * KF_SS_N);
     ck.resize(chunks * per * KF_CK_N);
-    carry_i.resize(6 * n);
-    carry_d.resize(30 * n);
+    carry_i.resize(KF_CARRY_I * n);
+    carry_d.resize(KF_CARRY_D * n);
     if (mixed) {
       ssf.resize(chunks * per * KF_SS_N);
       a.tss = ssf.data();

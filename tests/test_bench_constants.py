@@ -45,7 +45,7 @@ def test_bytes_per_track_constants():
     assert (np.diff(b.hit_offset) == 10).all() and (b.seq_count == 10).all()
     hits = 10 * (4 + 2 * 8 + 3 * 8)
     inputs = 6 * 8 + 21 * 8 + 3 * 8 + 2 * 4 + 8 + hits
-    carry = 6 * 4 + 30 * 8
+    carry = 8 * 4 + 30 * 8
     outputs = 6 * 8 + 21 * 8 + 4
     assert ext["forward"] == inputs + 16
     assert ext["backward"] == 48 + outputs

@@ -92,6 +92,36 @@ def test_fp64_ragged_grid_holes_material(torch_cuda, cfg, n):
     check_parity(g, o)
 
 
+@pytest.mark.parametrize("sched", [dict(), dict(sort_by_length=1)])
+def test_fp64_fewer_state_slots_than_planes(torch_cuda, sched):
+    """Config 3 (5-30 planes, 5 % holes) with room for 22 states: a sequence
+    of up to 22 planes keeps its states in the slots of its planes'
+    positions, a longer one in the slots of its measurement indices (and a
+    track that measures more than 22 states fails, as the oracle's fixed
+    state capacity); FP64 within 1e-9 of the oracle, track states included,
+    with and without the length sort."""
+    from gpukalmanfitter_amd.fitter import DeviceBatch, KalmanFitter, KalmanFitterOptions
+    import torch
+    b = G.generate(3, n_tracks=3000)
+    assert (b.seq_count > 22).any() and (b.seq_count <= 22).any()
+    kf = KalmanFitter(b.planes, b.field)
+    if sched:
+        kf.set_schedule(**sched)
+    dev = DeviceBatch(b, track_states=True, max_states=22)
+    kf.fit_batch(dev, KalmanFitterOptions(max_states=22, **W))
+    torch.cuda.synchronize()
+    g = dev.results_numpy()
+    kf.close()
+    o, _, _, _ = oracle.fit(b, c_options(max_states=22, **W), track_states=True)
+    long_ok = (b.seq_count > 22) & (np.diff(b.hit_offset) <= 22)
+    assert long_ok.any() and ((o.status[long_ok] & abi.KF_STATUS_OK) != 0).all()
+    check_parity(g, o)
+    smoothed = ((o.status & abi.KF_STATUS_SMOOTHED) != 0) & ((o.status & abi.KF_STATUS_SMOOTHER_NAN) == 0)
+    worst = compare_states(g["track_states"], o.track_states, o.n_measured, smoothed)
+    for k, v in worst.items():
+        assert v <= (0 if k == "geo_id" else TOL_FP64), (k, v)
+
+
 def test_fp64_misaligned_detector_and_field_scale(torch_cuda):
     b = G.generate(4, n_tracks=2000)
     # fit with the nominal planes and a non-unit field scale

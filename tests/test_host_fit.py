@@ -100,3 +100,26 @@ def test_host_mixed_fit_within_bars(built, cfg):
     assert np.array_equal(h["status"], o.status)
     pd, cd, c2 = compare_fit(h, o)
     assert pd <= 1e-5 and cd <= 1e-3, (pd, cd)
+
+
+@pytest.mark.parametrize("precision", [abi.KF_PRECISION_FP64, abi.KF_PRECISION_MIXED])
+def test_host_fit_with_fewer_state_slots_than_planes(built, precision):
+    """Config 3 (5-30 planes, 5 % holes) with room for 22 states: tracks
+    whose sequence fits 22 slots store their states by plane position, the
+    longer ones by measurement index (and fail when they measure more than
+    22 states, as the oracle's fixed state capacity does)."""
+    b = G.generate(3, n_tracks=3000)
+    opt = abi.default_options(max_states=22, precision=precision, **W)
+    assert (b.seq_count > 22).any() and (b.seq_count <= 22).any()
+    h = host_fit(b, opt)
+    o, _, _, _ = oracle.fit(b, abi.default_options(max_states=22, **W))
+    assert np.array_equal(h["status"], o.status)
+    assert np.array_equal(h["n_measured"], o.n_measured)
+    nh = np.diff(b.hit_offset)
+    long_ok = (b.seq_count > 22) & (nh <= 22)
+    assert long_ok.any() and ((o.status[long_ok] & abi.KF_STATUS_OK) != 0).all()
+    pd, cd, c2 = compare_fit(h, o)
+    if precision == abi.KF_PRECISION_FP64:
+        assert max(pd, cd, c2) <= 1e-9, (pd, cd, c2)
+    else:
+        assert pd <= 1e-5 and cd <= 1e-3, (pd, cd)

@@ -48,7 +48,7 @@ def test_noalloc_equals_fit_on_two_calls(torch_cuda):
     big = G.generate(5, n_tracks=30000)
     small = G.generate(5, n_tracks=2000, seed=99)
     kf = KalmanFitter(big.planes, big.field)
-    per_track = big.max_states * 512 + 264
+    per_track = big.max_states * 512 + 272
     assert 30000 * per_track > abi.KF_NOALLOC_WORKSPACE_BYTES  # several slices
     print(f"[noalloc] call 1: {big.n_tracks} tracks fp64", flush=True)
     ref, got = fit_both(kf, big, "fp64")
