@@ -5,7 +5,7 @@ calling it is done by :mod:`gpukalmanfitter_amd.fitter`.
 """
 import ctypes as C
 
-KF_ABI_VERSION = 3
+KF_ABI_VERSION = 4
 KF_NOALLOC_WORKSPACE_BYTES = 256 << 20
 
 KF_OK = 0
@@ -108,6 +108,7 @@ class kf_schedule(C.Structure):
         ("chunks_per_grab", C.c_int32),
         ("lanes_per_track", C.c_int32),
         ("graph", C.c_int32),
+        ("store_by_plane", C.c_int32),
     ]
 
     def as_dict(self):

@@ -556,6 +556,7 @@ int kf_fit(kf_ctx *ctx, const kf_tracks *tr, const kf_options *opt, const kf_res
   const kf_schedule &sc = ctx->sched;
   KfdLaunch L{};
   L.lanes_per_track = sc.lanes_per_track;
+  L.store_by_plane = sc.store_by_plane;
   if (sc.sort_by_length || sc.persistent) {
     const size_t order_bytes = ((size_t)a.n * sizeof(int32_t) + 255) / 256 * 256;
     rc = grow(ctx, &ctx->sched_buf, &ctx->sched_bytes, order_bytes + 514 * sizeof(uint32_t), s,
@@ -631,6 +632,7 @@ void kf_schedule_default(kf_schedule *s) {
   s->chunks_per_grab = 1;
   s->lanes_per_track = 1;
   s->graph = 0;
+  s->store_by_plane = 0;
 }
 
 int kf_set_schedule(kf_ctx *ctx, const kf_schedule *s) {
@@ -638,6 +640,11 @@ int kf_set_schedule(kf_ctx *ctx, const kf_schedule *s) {
   if (!s) return fail(ctx, KF_E_INVALID, "kf_set_schedule: schedule is NULL");
   if (s->lanes_per_track != 1 && s->lanes_per_track != 2 && s->lanes_per_track != 4)
     return fail(ctx, KF_E_INVALID, "kf_set_schedule: lanes_per_track must be 1, 2 or 4");
+  if (s->store_by_plane != 0 && s->store_by_plane != 1)
+    return fail(ctx, KF_E_INVALID, "kf_set_schedule: store_by_plane must be 0 or 1");
+  if (s->store_by_plane && s->lanes_per_track > 1)
+    return fail(ctx, KF_E_INVALID,
+                "kf_set_schedule: store_by_plane runs with one lane per track");
   if (s->lanes_per_track > 1 && s->persistent)
     return fail(ctx, KF_E_INVALID,
                 "kf_set_schedule: lanes_per_track > 1 runs on the one-shot grid (persistent = 0)");
@@ -661,25 +668,28 @@ int kf_autotune(kf_ctx *ctx, const kf_tracks *tracks, const kf_options *opt, con
   if (!tracks || !opt || !res) return fail(ctx, KF_E_INVALID, "kf_autotune: NULL argument");
   if (repeats < 1) repeats = 1;
   std::vector<kf_schedule> cand;
-  for (int sort = 0; sort <= 1; ++sort) {
-    kf_schedule c;
-    kf_schedule_default(&c);
-    c.sort_by_length = sort;
-    c.graph = 1;
-    cand.push_back(c);
-    for (int l : {2, 4}) {  // lane groups (one-shot grid)
-      kf_schedule cl = c;
-      cl.lanes_per_track = l;
-      cand.push_back(cl);
+  for (int sort = 0; sort <= 1; ++sort)
+    for (int byPlane = 0; byPlane <= 1; ++byPlane) {
+      kf_schedule c;
+      kf_schedule_default(&c);
+      c.sort_by_length = sort;
+      c.store_by_plane = byPlane;
+      c.graph = 1;
+      cand.push_back(c);
+      if (!byPlane)
+        for (int l : {2, 4}) {  // lane groups (one-shot grid, slots by measurement)
+          kf_schedule cl = c;
+          cl.lanes_per_track = l;
+          cand.push_back(cl);
+        }
+      for (int w : {2, 4, 8})
+        for (int g : {1, 4}) {
+          c.persistent = 1;
+          c.waves_per_cu = w;
+          c.chunks_per_grab = g;
+          cand.push_back(c);
+        }
     }
-    for (int w : {2, 4, 8})
-      for (int g : {1, 4}) {
-        c.persistent = 1;
-        c.waves_per_cu = w;
-        c.chunks_per_grab = g;
-        cand.push_back(c);
-      }
-  }
   const kf_schedule saved = ctx->sched;
   double best_t = 1e300;
   kf_schedule best_s = saved;

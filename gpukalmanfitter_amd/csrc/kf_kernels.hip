@@ -27,14 +27,14 @@ struct ChunkBases {
 
 // Fits the track at position pos (pass FWD of the two kernels); `rank` is
 // the lane's index in the track's group of L lanes.
-template <typename S, bool FWD, int L>
+template <typename S, bool FWD, int L, bool PS>
 KFD_INLINE void fitPosition(const FitArgs &a, int64_t local, uint32_t rank,
                             const DevPlane *planes) {
   const ChunkBases<S> cb(a, local);
   if (local >= a.count) return;
   const int64_t pos = a.first + local;
   const int64_t i = a.order ? (int64_t)a.order[pos] : pos;
-  TrackFit<S, FWD, L> tf(a, i, cb.d, cb.s, cb.f, cb.c, (uint32_t)(local & 63), local);
+  TrackFit<S, FWD, L, PS> tf(a, i, cb.d, cb.s, cb.f, cb.c, (uint32_t)(local & 63), local);
   tf.setRank(rank);
   tf.pls = planes;
   if constexpr (FWD)
@@ -54,9 +54,12 @@ KFD_INLINE void fitPosition(const FitArgs &a, int64_t local, uint32_t rank,
 // queue loop's registers).
 // L: lanes per track (one-shot grid only): L consecutive lanes fit one
 // position, a wave covers 64 / L positions of one 64-position store chunk.
-template <typename S, bool FWD, bool LDS, bool PERS, int L>
+// PS: store slots by plane position (kf_schedule.store_by_plane; one lane
+// per track).
+template <typename S, bool FWD, bool LDS, bool PERS, int L, bool PS>
 __global__ void __launch_bounds__(KF_BLOCK) kf_fit_kernel(FitArgs args) {
   static_assert(L == 1 || !PERS, "lane groups run on the one-shot grid");
+  static_assert(L == 1 || !PS, "store slots by plane with one lane per track");
   const DevPlane *planes = args.planes;
   if constexpr (LDS) {
     __shared__ DevPlane lds_planes[KF_LDS_PLANES];
@@ -70,7 +73,7 @@ __global__ void __launch_bounds__(KF_BLOCK) kf_fit_kernel(FitArgs args) {
   const uint32_t lane = threadIdx.x & 63u;
   if constexpr (!PERS) {
     const int64_t t = (int64_t)blockIdx.x * KF_BLOCK + threadIdx.x;
-    fitPosition<S, FWD, L>(args, t / L, (uint32_t)(t % L), planes);
+    fitPosition<S, FWD, L, PS>(args, t / L, (uint32_t)(t % L), planes);
   } else {
     const uint32_t nchunks = (uint32_t)((args.count + 63) >> 6);
     const uint32_t grab = (uint32_t)args.chunks_per_grab;
@@ -82,7 +85,7 @@ __global__ void __launch_bounds__(KF_BLOCK) kf_fit_kernel(FitArgs args) {
       if (c0 >= nchunks) break;
       const uint32_t c1 = (c0 + grab < nchunks) ? c0 + grab : nchunks;
       for (uint32_t c = c0; c < c1; ++c)
-        fitPosition<S, FWD, 1>(args, (int64_t)c * 64 + lane, 0u, planes);
+        fitPosition<S, FWD, 1, PS>(args, (int64_t)c * 64 + lane, 0u, planes);
     }
   }
 }
@@ -175,26 +178,31 @@ extern "C" hipError_t kfd_launch_fit(const kfd::FitArgs *args, int precision, hi
   const bool pers = launch && launch->persistent_blocks > 0;
   const int lpt = (launch && !pers && launch->lanes_per_track > 1) ? launch->lanes_per_track : 1;
   if (lpt != 1 && lpt != 2 && lpt != 4) return hipErrorInvalidValue;
+  const bool ps = launch && launch->store_by_plane != 0;
+  if (ps && lpt != 1) return hipErrorInvalidValue;
   if (pers) hipLaunchKernelGGL(kfd::kf_queue_reset, dim3(1), dim3(64), 0, stream, args->queue);
   const int64_t blocks =
       pers ? launch->persistent_blocks : (args->count * lpt + KF_BLOCK - 1) / KF_BLOCK;
   const dim3 grid((unsigned)blocks), block(KF_BLOCK);
   const bool lds = args->n_planes <= KF_LDS_PLANES;
   const bool fp64 = precision == KF_PRECISION_FP64;
-#define KFD_LAUNCH1(S_, FWD_, LDS_)                                                              \
-  do {                                                                                          \
-    if (pers)                                                                                   \
-      hipLaunchKernelGGL((kfd::kf_fit_kernel<S_, FWD_, LDS_, true, 1>), grid, block, 0, stream,   \
-                         *args);                                                                \
-    else if (lpt == 1)                                                                          \
-      hipLaunchKernelGGL((kfd::kf_fit_kernel<S_, FWD_, LDS_, false, 1>), grid, block, 0, stream,  \
-                         *args);                                                                \
-    else if (lpt == 2)                                                                          \
-      hipLaunchKernelGGL((kfd::kf_fit_kernel<S_, FWD_, LDS_, false, 2>), grid, block, 0, stream,  \
-                         *args);                                                                \
-    else                                                                                        \
-      hipLaunchKernelGGL((kfd::kf_fit_kernel<S_, FWD_, LDS_, false, 4>), grid, block, 0, stream,  \
-                         *args);                                                                \
+#define KFD_LAUNCH2(S_, FWD_, LDS_, PERS_, L_, PS_)                                            \
+  hipLaunchKernelGGL((kfd::kf_fit_kernel<S_, FWD_, LDS_, PERS_, L_, PS_>), grid, block, 0,    \
+                     stream, *args)
+#define KFD_LAUNCH1(S_, FWD_, LDS_)                                  \
+  do {                                                              \
+    if (pers && ps)                                                 \
+      KFD_LAUNCH2(S_, FWD_, LDS_, true, 1, true);                   \
+    else if (pers)                                                  \
+      KFD_LAUNCH2(S_, FWD_, LDS_, true, 1, false);                  \
+    else if (lpt == 1 && ps)                                        \
+      KFD_LAUNCH2(S_, FWD_, LDS_, false, 1, true);                  \
+    else if (lpt == 1)                                              \
+      KFD_LAUNCH2(S_, FWD_, LDS_, false, 1, false);                 \
+    else if (lpt == 2)                                              \
+      KFD_LAUNCH2(S_, FWD_, LDS_, false, 2, false);                 \
+    else                                                            \
+      KFD_LAUNCH2(S_, FWD_, LDS_, false, 4, false);                 \
   } while (0)
 #define KFD_LAUNCH(S_, FWD_)          \
   do {                                \
@@ -217,6 +225,7 @@ extern "C" hipError_t kfd_launch_fit(const kfd::FitArgs *args, int precision, hi
     KFD_LAUNCH(float, false);
 #undef KFD_LAUNCH
 #undef KFD_LAUNCH1
+#undef KFD_LAUNCH2
   if (track_states_out) {
     const int64_t eb = (args->count + 255) / 256;
     hipLaunchKernelGGL(kfd::kf_export_states, dim3((unsigned)eb), dim3(256), 0, stream, *args,

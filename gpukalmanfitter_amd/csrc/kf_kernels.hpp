@@ -84,8 +84,8 @@ struct FitArgs {
   uint32_t *out_status;
   // Per-state store of the smoother, chunked by 64 track positions and kept
   // in 16-byte groups per lane (W = 16 / sizeof(element) elements): element
-  // f of state k of position i sits at
-  //   base[(((i / 64) * max_states * F + k * F + f - f % W) * 64 + (i % 64) * W) + f % W],
+  // f of the state in slot s (KF_SD_* above) of position i sits at
+  //   base[(((i / 64) * max_states * F + s * F + f - f % W) * 64 + (i % 64) * W) + f % W],
   // so one 16-byte vector access per lane moves W elements and a wave's
   // access is 1 KB contiguous.
   //  tsd  double, F = KF_SD_N: predicted and filtered parameters;
@@ -96,7 +96,8 @@ struct FitArgs {
   //       update;
   //  tsf  double, F = KF_TS_STRIDE: the public track-state record, written
   //       only when the caller asks for track states (else NULL), one
-  //       element per lane: base[((i / 64) * max_states * F + k * F + f) * 64 + i % 64].
+  //       element per lane, by measurement index k:
+  //       base[((i / 64) * max_states * F + k * F + f) * 64 + i % 64].
   // The predicted covariance is not stored: the smoother recomputes it from
   // the stored Jacobian and post-material covariance with the same operation
   // sequence, so it is bit-identical to the forward pass's.
@@ -155,6 +156,7 @@ struct KfdLaunch {
   int32_t sort_by_length;  // order positions by decreasing sequence length first
   int32_t persistent_blocks;  // > 0: persistent grid of this many blocks
   int32_t lanes_per_track;  // 1, 2 or 4 lanes per position (one-shot grid only)
+  int32_t store_by_plane;   // store slots by plane position (one lane per position)
   uint32_t *sort_scratch;  // 256 + 256 counters (length histogram, bin offsets)
   int32_t *order;          // [n] positions -> tracks (when sorting)
 };

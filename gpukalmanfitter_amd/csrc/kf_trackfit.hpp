@@ -29,7 +29,7 @@ enum { kResumeNone = 0, kResumePreLoop = 1, kResumeInLoop = 2, kResumePostLoop =
 // group run this program on the same data; the free transport matrix of each
 // RK step is built column by column across them, and only the group's first
 // lane writes to memory.
-template <typename S, bool FWD, int L = 1>
+template <typename S, bool FWD, int L = 1, bool PS = false>
 struct TrackFit {
   static constexpr bool kMixed = sizeof(S) < sizeof(double);
   const FitArgs &a;
@@ -74,11 +74,13 @@ struct TrackFit {
   int measured;
   int outliers;    // measured states the outlier finder left without an update
   int firstPlane;  // plane-table index of the first measured state
-  // store slots of the last two measured states (KF_SD_* / KF_SS_*: the slot
-  // of a state is its plane's position in the sequence when the sequence
-  // fits max_states slots, else its measurement index)
-  int slotLast, slotPrev;
-  KFD_INLINE bool slotByPlane() const { return size <= a.max_states; }
+  // store slots (KF_SD_* / KF_SS_*).  PS = false: the slot of measured state
+  // k is k.  PS = true (kf_schedule.store_by_plane): k + min(holes before
+  // it, max_states - hits), i.e. its plane's position in the sequence
+  // whenever the sequence fits max_states slots.  holes counts the planes
+  // passed without a hit; slotLast / slotPrev are the slots of the last two
+  // measured states.
+  int holes, slotLast, slotPrev;
   bool smoothed, finished, result, smootherNaN;
   int64_t h0, h1;
   // hit search: while every match so far was the next hit in order and the
@@ -1088,7 +1090,10 @@ struct TrackFit {
         for (int q = 0; q < 4; ++q)
           if (hit < 0 && h + q < h1 && g[q] == pl.geo_id) hit = h + q;
       }
-      if (hit < 0) return true;  // hole
+      if (hit < 0) {  // hole
+        if constexpr (PS) ++holes;
+        return true;
+      }
       m0 = a.hit_meas[hit];
       m1 = a.hit_meas[a.n_hits + hit];
       V00 = a.hit_cov[hit];
@@ -1102,7 +1107,11 @@ struct TrackFit {
     }
     if (measured >= a.max_states) return false;
     const int k = measured;
-    const int s = slotByPlane() ? sp - seqFirst : k;  // store slot
+    int s = k;  // store slot
+    if constexpr (PS) {
+      const int slack = a.max_states - (int)(h1 - h0);
+      s = k + ((holes < slack) ? holes : (slack > 0 ? slack : 0));
+    }
     if (k == 0) firstPlane = sp;
     double xp[6];
     S jb[6][6];
@@ -1117,7 +1126,7 @@ struct TrackFit {
 #pragma unroll
         for (int c = 0; c < 5; ++c)
           if (r != 4) jv[jidx(r, c)] = jb[r][c];
-      jv[KF_SS_LINK] = linkBits(slotPrev, (S)0);
+      jv[KF_SS_LINK] = PS ? linkBits(slotPrev, (S)0) : (S)0;
 #pragma unroll
       for (int m = KF_SS_LINK + 1; m < KF_SS_CF - KF_SS_J; ++m) jv[m] = (S)0;
       storeS<KF_SS_CF - KF_SS_J>(s, KF_SS_J, jv);
@@ -1321,14 +1330,111 @@ struct TrackFit {
     loadPar(sn, KF_SD_XP, in.xpn);
   }
 
-  // GainMatrixSmoother (GainMatrixSmoother.hpp:21-94):
+  // One smoother step (GainMatrixSmoother.hpp:21-94): state k from its
+  // stored operands `in` and the smoothed state k + 1 (xsn, csn), which it
+  // overwrites with the smoothed state k.  False on a NaN gain.
   // G = (C_f J^T) C_p^-1, x_s = x_f + G (x_s' - x_p'), C_s = C_f - G (C_p - C_s') G^T,
-  // in double in both modes.  The operands of step k - 1 are loaded while
-  // step k computes.  The loop walks the store slots downwards from the last
+  // in double in both modes.
+  KFD_INLINE bool smoothStep(int k, const SmoothIn &in, double xsn[6], double csn[21]) {
+    S J[6][6];
+#pragma unroll
+    for (int r = 0; r < 6; ++r)
+#pragma unroll
+      for (int c = 0; c < 6; ++c)
+        J[r][c] = (r == 4 || c == 5) ? ((r == c) ? (S)1 : (S)0) : in.j[jidx(r, c)];
+    // predicted covariance of state k + 1: the forward pass's transport of
+    // the post-material covariance of state k, from the same operands
+    double Cp[21];
+    {
+      S cm[21];
+#pragma unroll
+      for (int m = 0; m < 21; ++m) cm[m] = in.cf[m];
+      cm[sidx(2, 2)] = in.md[0];
+      cm[sidx(3, 3)] = in.md[1];
+      cm[sidx(4, 4)] = in.md[2];
+      sandwich(J, cm, Cp);
+    }
+    double cf[21];
+#pragma unroll
+    for (int m = 0; m < 21; ++m) cf[m] = (double)in.cf[m];
+    // G^T = C_p^-1 (J C_f) (C_f, C_p symmetric); row 4 of J is e4 and
+    // column 5 is e5
+    double G[6][6];
+    {
+      double X[6][6];
+#pragma unroll
+      for (int r = 0; r < 6; ++r)
+#pragma unroll
+        for (int c = 0; c < 6; ++c) {
+          double s;
+          if (r == 4) {
+            s = cf[sidx(4, c)];
+          } else {
+            s = (double)J[r][0] * cf[sidx(0, c)];
+#pragma unroll
+            for (int m = 1; m < 5; ++m) s = s + (double)J[r][m] * cf[sidx(m, c)];
+            if (r == 5) s = s + cf[sidx(5, c)];
+          }
+          X[r][c] = s;
+        }
+      spdSolve(Cp, X);
+#pragma unroll
+      for (int r = 0; r < 6; ++r)
+#pragma unroll
+        for (int c = 0; c < 6; ++c) G[r][c] = X[c][r];
+    }
+    bool nan = false;  // GainMatrixSmoother.hpp:63-66
+#pragma unroll
+    for (int r = 0; r < 6; ++r)
+#pragma unroll
+      for (int c = 0; c < 6; ++c) nan = nan || (G[r][c] != G[r][c]);
+    if (nan) {
+      smootherNaN = true;
+      return false;
+    }
+    double dx[6];
+#pragma unroll
+    for (int r = 0; r < 6; ++r) dx[r] = xsn[r] - in.xpn[r];
+#pragma unroll
+    for (int r = 0; r < 6; ++r) {
+      double s = G[r][0] * dx[0];
+#pragma unroll
+      for (int m = 1; m < 6; ++m) s = s + G[r][m] * dx[m];
+      xsn[r] = in.xf[r] + s;
+    }
+    double Dm[21];
+#pragma unroll
+    for (int m = 0; m < 21; ++m) Dm[m] = Cp[m] - csn[m];
+#pragma unroll
+    for (int r = 0; r < 6; ++r) {
+      double B1[6];  // row r of G (C_p - C_s')
+#pragma unroll
+      for (int c = 0; c < 6; ++c) {
+        double s = G[r][0] * Dm[sidx(0, c)];
+#pragma unroll
+        for (int m = 1; m < 6; ++m) s = s + G[r][m] * Dm[sidx(m, c)];
+        B1[c] = s;
+      }
+#pragma unroll
+      for (int c = r; c < 6; ++c) {
+        double s = B1[0] * G[c][0];
+#pragma unroll
+        for (int m = 1; m < 6; ++m) s = s + B1[m] * G[c][m];
+        csn[sidx(r, c)] = cf[sidx(r, c)] - s;
+      }
+    }
+    correctParams(xsn);
+    if (tsf) storeBoundF(k, KF_TS_SMOOTHED, xsn, csn);
+    return true;
+  }
+
+  // The smoother over the stored states.  The operands of step k - 1 are
+  // loaded while step k computes.  PS = false: the loop runs over k (state k
+  // in slot k).  PS = true: it walks the store slots downwards from the last
   // one the sequence can use and smooths a state when it reaches its slot,
-  // so the lanes of a wave whose tracks cross the same planes read the same
-  // store chunk in the same iteration, holes or not (a lane idles at the
-  // slots of its holes); it ends after at most max_states iterations.
+  // so the lanes whose tracks cross the same planes read the same store
+  // chunk in the same iteration (a lane idles at the slots of its holes);
+  // the walk ends after at most max_states iterations.
   KFD_INLINE bool smooth(double xs[6], double cs[21]) {
     const int n = measured;
     double xsn[6], csn[21];
@@ -1337,110 +1443,33 @@ struct TrackFit {
     loadFilteredCov(slotLast, csn);
     if (tsf) storeBoundF(n - 1, KF_TS_SMOOTHED, xsn, csn);
     SmoothIn cur;
-    int sk = slotPrev;  // slot of state k
-    if (n >= 2) loadSmoothIn(sk, slotLast, cur);
-    int slot = slotByPlane() ? size - 1 : sk;
-    for (int k = n - 2; k >= 0; --slot) {
-      if (slot < 0 || sk < 0) {  // slots out of order: the store is inconsistent
+    if constexpr (!PS) {
+      if (n >= 2) loadSmoothIn(n - 2, n - 1, cur);
+      for (int k = n - 2; k >= 0; --k) {
+        SmoothIn nxt;
+        loadSmoothIn((k > 0) ? k - 1 : 0, k, nxt);
+        if (!smoothStep(k, cur, xsn, csn)) return false;
+        cur = nxt;
+      }
+    } else {
+      int sk = slotPrev;  // slot of state k
+      if (n >= 2) loadSmoothIn(sk, slotLast, cur);
+      int slot = ((size < a.max_states) ? size : a.max_states) - 1;
+      int k = n - 2;
+      for (; k >= 0 && slot >= 0; --slot) {
+        if (slot != sk) continue;  // a hole of this track
+        const int link = cur.link;
+        SmoothIn nxt;
+        loadSmoothIn((k > 0 && link >= 0 && link < sk) ? link : sk, sk, nxt);
+        if (!smoothStep(k, cur, xsn, csn)) return false;
+        cur = nxt;
+        sk = link;
+        --k;
+      }
+      if (k >= 0) {  // slots out of order: the store is inconsistent
         smootherNaN = true;
         return false;
       }
-      if (slot != sk) continue;  // a hole of this track
-      const int link = cur.link;
-      SmoothIn nxt;
-      loadSmoothIn((k > 0 && link >= 0 && link < sk) ? link : sk, sk, nxt);
-      S J[6][6];
-#pragma unroll
-      for (int r = 0; r < 6; ++r)
-#pragma unroll
-        for (int c = 0; c < 6; ++c)
-          J[r][c] = (r == 4 || c == 5) ? ((r == c) ? (S)1 : (S)0) : cur.j[jidx(r, c)];
-      // predicted covariance of state k + 1: the forward pass's transport of
-      // the post-material covariance of state k, from the same operands
-      double Cp[21];
-      {
-        S cm[21];
-#pragma unroll
-        for (int m = 0; m < 21; ++m) cm[m] = cur.cf[m];
-        cm[sidx(2, 2)] = cur.md[0];
-        cm[sidx(3, 3)] = cur.md[1];
-        cm[sidx(4, 4)] = cur.md[2];
-        sandwich(J, cm, Cp);
-      }
-      double cf[21];
-#pragma unroll
-      for (int m = 0; m < 21; ++m) cf[m] = (double)cur.cf[m];
-      // G^T = C_p^-1 (J C_f) (C_f, C_p symmetric); row 4 of J is e4 and
-      // column 5 is e5
-      double G[6][6];
-      {
-        double X[6][6];
-#pragma unroll
-        for (int r = 0; r < 6; ++r)
-#pragma unroll
-          for (int c = 0; c < 6; ++c) {
-            double s;
-            if (r == 4) {
-              s = cf[sidx(4, c)];
-            } else {
-              s = (double)J[r][0] * cf[sidx(0, c)];
-#pragma unroll
-              for (int m = 1; m < 5; ++m) s = s + (double)J[r][m] * cf[sidx(m, c)];
-              if (r == 5) s = s + cf[sidx(5, c)];
-            }
-            X[r][c] = s;
-          }
-        spdSolve(Cp, X);
-#pragma unroll
-        for (int r = 0; r < 6; ++r)
-#pragma unroll
-          for (int c = 0; c < 6; ++c) G[r][c] = X[c][r];
-      }
-      bool nan = false;  // GainMatrixSmoother.hpp:63-66
-#pragma unroll
-      for (int r = 0; r < 6; ++r)
-#pragma unroll
-        for (int c = 0; c < 6; ++c) nan = nan || (G[r][c] != G[r][c]);
-      if (nan) {
-        smootherNaN = true;
-        return false;
-      }
-      double dx[6];
-#pragma unroll
-      for (int r = 0; r < 6; ++r) dx[r] = xsn[r] - cur.xpn[r];
-#pragma unroll
-      for (int r = 0; r < 6; ++r) {
-        double s = G[r][0] * dx[0];
-#pragma unroll
-        for (int m = 1; m < 6; ++m) s = s + G[r][m] * dx[m];
-        xsn[r] = cur.xf[r] + s;
-      }
-      double Dm[21];
-#pragma unroll
-      for (int m = 0; m < 21; ++m) Dm[m] = Cp[m] - csn[m];
-#pragma unroll
-      for (int r = 0; r < 6; ++r) {
-        double B1[6];  // row r of G (C_p - C_s')
-#pragma unroll
-        for (int c = 0; c < 6; ++c) {
-          double s = G[r][0] * Dm[sidx(0, c)];
-#pragma unroll
-          for (int m = 1; m < 6; ++m) s = s + G[r][m] * Dm[sidx(m, c)];
-          B1[c] = s;
-        }
-#pragma unroll
-        for (int c = r; c < 6; ++c) {
-          double s = B1[0] * G[c][0];
-#pragma unroll
-          for (int m = 1; m < 6; ++m) s = s + B1[m] * G[c][m];
-          csn[sidx(r, c)] = cf[sidx(r, c)] - s;
-        }
-      }
-      correctParams(xsn);
-      if (tsf) storeBoundF(k, KF_TS_SMOOTHED, xsn, csn);
-      cur = nxt;
-      sk = (k > 0) ? link : 0;
-      --k;
     }
 #pragma unroll
     for (int m = 0; m < 6; ++m) xs[m] = xsn[m];
@@ -1647,6 +1676,7 @@ struct TrackFit {
     measured = 0;
     outliers = 0;
     firstPlane = 0;
+    holes = 0;
     slotLast = slotPrev = -1;
     result = true;
     chi2sum = 0.0;

@@ -58,7 +58,7 @@
 extern "C" {
 #endif
 
-#define KF_ABI_VERSION 3
+#define KF_ABI_VERSION 4
 
 /* return codes */
 #define KF_OK 0
@@ -180,6 +180,11 @@ typedef struct kf_schedule {
                               1 (default), 2 or 4 consecutive lanes, one-shot grid only */
   int32_t graph;           /* 1: capture the launch sequence as a hipGraph on the first
                               call and replay it while the arguments are unchanged */
+  int32_t store_by_plane;  /* 1: keep a measured state in the smoother store at its
+                              plane's position in the sequence rather than at its
+                              measurement index, so the tracks of a wave write and read
+                              one store chunk per plane when they have holes (for
+                              batches with holes; one lane per track; default 0) */
 } kf_schedule;
 
 typedef struct kf_ctx kf_ctx;
@@ -265,12 +270,13 @@ void kf_schedule_default(kf_schedule *s);
 int kf_set_schedule(kf_ctx *ctx, const kf_schedule *s);
 int kf_get_schedule(const kf_ctx *ctx, kf_schedule *s);
 
-/* Autotuner: fits the batch with each candidate schedule (sort on/off,
- * one-shot grid or persistent work queue with 2/4/8 waves per CU and 1/4
- * chunks per grab, graph replay), times every candidate on `stream` with HIP
- * events (best of `repeats` fits), keeps the fastest as ctx's schedule and
- * returns it with its time.  The results buffers hold the last fit.  The
- * one-shot candidates also run with 2 and 4 lanes per track. */
+/* Autotuner: fits the batch with each candidate schedule (sort on/off x
+ * store slots by measurement or by plane x one-shot grid or persistent work
+ * queue with 2/4/8 waves per CU and 1/4 chunks per grab, graph replay), times
+ * every candidate on `stream` with HIP events (best of `repeats` fits), keeps
+ * the fastest as ctx's schedule and returns it with its time.  The results
+ * buffers hold the last fit.  The one-shot candidates with slots by
+ * measurement also run with 2 and 4 lanes per track. */
 int kf_autotune(kf_ctx *ctx, const kf_tracks *tracks, const kf_options *opt,
                 const kf_results *res, void *stream, int32_t repeats, kf_schedule *best,
                 double *best_ms);

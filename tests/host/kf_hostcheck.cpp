@@ -138,7 +138,7 @@ struct Host {
   }
 
   // The whole fit, both kernels' programs one track at a time.
-  template <typename S>
+  template <typename S, bool PS>
   void fit() {
     a.tsc = nullptr;
     for (int pass = 0; pass < 2; ++pass)
@@ -148,11 +148,11 @@ struct Host {
         S *s = (S *)a.tss + ch * per * KF_SS_N;
         double *f = a.tsf ? a.tsf + ch * per * KF_TS_STRIDE : nullptr;
         if (pass == 0) {
-          TrackFit<S, true> tf(a, i, d, s, f, nullptr, (uint32_t)(i & 63), i);
+          TrackFit<S, true, 1, PS> tf(a, i, d, s, f, nullptr, (uint32_t)(i & 63), i);
           tf.pls = planes.data();
           tf.runForward();
         } else {
-          TrackFit<S, false> tf(a, i, d, s, f, nullptr, (uint32_t)(i & 63), i);
+          TrackFit<S, false, 1, PS> tf(a, i, d, s, f, nullptr, (uint32_t)(i & 63), i);
           tf.pls = planes.data();
           tf.runBackward();
         }
@@ -164,18 +164,19 @@ struct Host {
 
 extern "C" {
 
-// The fit of every track by the kernels' per-track program (FP64 or mixed):
+// The fit of every track by the kernels' per-track program (FP64 or mixed;
+// store slots by measurement or, with by_plane, by plane position):
 // params [6][n], cov [21][n], chi2, ndf, n_measured, status [n], and, when
 // track_states is not NULL, the track-state records [max_states][KF_TS_STRIDE][n].
 int kfh_fit(const kf_plane *planes, int32_t n_planes, const kfh_field *f, const kf_tracks *tr,
-            const kf_options *opt, double *params, double *cov, double *chi2, int32_t *ndf,
-            int32_t *n_measured, uint32_t *status, double *track_states) {
+            const kf_options *opt, int32_t by_plane, double *params, double *cov, double *chi2,
+            int32_t *ndf, int32_t *n_measured, uint32_t *status, double *track_states) {
   const bool mixed = opt->precision == KF_PRECISION_MIXED;
   Host h(planes, n_planes, f, tr, opt, true, track_states != nullptr, mixed);
   if (mixed)
-    h.fit<float>();
+    by_plane ? h.fit<float, true>() : h.fit<float, false>();
   else
-    h.fit<double>();
+    by_plane ? h.fit<double, true>() : h.fit<double, false>();
   const int64_t n = tr->n_tracks;
   std::memcpy(params, h.out_params.data(), 6 * n * sizeof(double));
   std::memcpy(cov, h.out_cov.data(), 21 * n * sizeof(double));

@@ -92,14 +92,16 @@ def test_fp64_ragged_grid_holes_material(torch_cuda, cfg, n):
     check_parity(g, o)
 
 
-@pytest.mark.parametrize("sched", [dict(), dict(sort_by_length=1)])
+@pytest.mark.parametrize("sched", [dict(), dict(sort_by_length=1), dict(store_by_plane=1),
+                                   dict(sort_by_length=1, store_by_plane=1)])
 def test_fp64_fewer_state_slots_than_planes(torch_cuda, sched):
-    """Config 3 (5-30 planes, 5 % holes) with room for 22 states: a sequence
-    of up to 22 planes keeps its states in the slots of its planes'
-    positions, a longer one in the slots of its measurement indices (and a
-    track that measures more than 22 states fails, as the oracle's fixed
-    state capacity); FP64 within 1e-9 of the oracle, track states included,
-    with and without the length sort."""
+    """Config 3 (5-30 planes, 5 % holes) with room for 22 states: with store
+    slots by plane, a sequence of up to 22 planes keeps its states in the
+    slots of its planes' positions and a longer one shifts them back into
+    the 22 slots; a track that measures more than 22 states fails, as the
+    oracle's fixed state capacity.  FP64 within 1e-9 of the oracle, track
+    states included, with either slot rule, with and without the length
+    sort."""
     from gpukalmanfitter_amd.fitter import DeviceBatch, KalmanFitter, KalmanFitterOptions
     import torch
     b = G.generate(3, n_tracks=3000)
@@ -403,14 +405,16 @@ def test_mixed_full_size_config2_pulls(torch_cuda):
 # ------------------------------------------------------------- scheduling --
 SCHEDULES = [dict(sort_by_length=1), dict(persistent=1, waves_per_cu=4, chunks_per_grab=1),
              dict(sort_by_length=1, persistent=1, waves_per_cu=8, chunks_per_grab=4, graph=1),
-             dict(graph=1)]
+             dict(graph=1), dict(store_by_plane=1),
+             dict(sort_by_length=1, store_by_plane=1, persistent=1, waves_per_cu=4, graph=1)]
 
 
 @pytest.mark.parametrize("sched", SCHEDULES)
 def test_schedules_give_identical_results(torch_cuda, sched):
-    """Length sort, persistent work queue and graph replay change where a
-    track runs, never its result: bitwise equal to the default schedule, and
-    the oracle on a ragged batch with holes (grid field)."""
+    """Length sort, persistent work queue, graph replay and store slots by
+    plane change where a track runs or keeps its states, never its result:
+    bitwise equal to the default schedule, and the oracle on a ragged batch
+    with holes (grid field)."""
     from gpukalmanfitter_amd.fitter import DeviceBatch, KalmanFitter, KalmanFitterOptions
     b = G.generate(3, n_tracks=5000)
     kf = KalmanFitter(b.planes, b.field)
@@ -430,6 +434,36 @@ def test_schedules_give_identical_results(torch_cuda, sched):
     kf.close()
     o, _, _, _ = oracle.fit(b, c_options(**W))
     check_parity(ref, o)
+
+
+@pytest.mark.parametrize("cfg", [3, 5])
+def test_store_by_plane_mixed_bitwise(torch_cuda, cfg):
+    """Mixed precision: store slots by plane (default and length-sorted
+    persistent schedules) give bitwise the results of the default schedule."""
+    from gpukalmanfitter_amd.fitter import DeviceBatch, KalmanFitter, KalmanFitterOptions
+    b = G.generate(cfg, n_tracks=4000)
+    kf = KalmanFitter(b.planes, b.field)
+    dev = DeviceBatch(b, track_states=True)
+    opts = KalmanFitterOptions(precision="mixed", **W)
+    kf.fit_batch(dev, opts)
+    ref = dev.results_numpy()
+    for sched in (dict(store_by_plane=1), dict(store_by_plane=1, sort_by_length=1, persistent=1)):
+        kf.set_schedule(**sched)
+        dev.clear_results()
+        kf.fit_batch(dev, opts)
+        got = dev.results_numpy()
+        for k in ("params", "cov", "chi2", "status", "n_measured", "track_states"):
+            assert np.array_equal(got[k], ref[k]), (sched, k)
+    kf.close()
+
+
+def test_store_by_plane_refused_with_lane_groups(torch_cuda):
+    from gpukalmanfitter_amd.fitter import FitError, KalmanFitter
+    b = G.generate(1, n_tracks=64)
+    kf = KalmanFitter(b.planes, b.field)
+    with pytest.raises(FitError, match="one lane per track"):
+        kf.set_schedule(store_by_plane=1, lanes_per_track=2)
+    kf.close()
 
 
 @pytest.mark.parametrize("lanes", [2, 4])

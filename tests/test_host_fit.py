@@ -33,12 +33,12 @@ def host_lib():
     return _LIB[0]
 
 
-def host_fit(b, opt, track_states=False):
+def host_fit(b, opt, track_states=False, by_plane=False):
     lib = host_lib()
     if not getattr(lib, "_fit_typed", False):
         P = C.c_void_p
         lib.kfh_fit.argtypes = [P, C.c_int32, C.c_void_p, C.POINTER(abi.kf_tracks),
-                                C.POINTER(abi.kf_options)] + [P] * 7
+                                C.POINTER(abi.kf_options), C.c_int32] + [P] * 7
         lib.kfh_fit.restype = C.c_int
         lib._fit_typed = True
     n = b.n_tracks
@@ -51,7 +51,7 @@ def host_fit(b, opt, track_states=False):
     fld = field_of(b, b.field.scale)
     ptr = lambda a: a.ctypes.data_as(C.c_void_p) if a is not None else None  # noqa: E731
     rc = lib.kfh_fit(b.planes, len(b.planes), C.byref(fld), C.byref(tr), C.byref(opt),
-                     *[ptr(out[k]) for k in ("params", "cov", "chi2", "ndf", "n_measured",
+                     int(by_plane), *[ptr(out[k]) for k in ("params", "cov", "chi2", "ndf", "n_measured",
                                              "status")], ptr(ts))
     assert rc == 0
     if track_states:
@@ -59,11 +59,12 @@ def host_fit(b, opt, track_states=False):
     return out
 
 
+@pytest.mark.parametrize("by_plane", [False, True])
 @pytest.mark.parametrize("cfg", [1, 3, 5])
-def test_host_fp64_fit_matches_oracle(built, cfg):
+def test_host_fp64_fit_matches_oracle(built, cfg, by_plane):
     b = G.generate(cfg, n_tracks=2000)
     opt = abi.default_options(**W)
-    h = host_fit(b, opt, track_states=True)
+    h = host_fit(b, opt, track_states=True, by_plane=by_plane)
     o, _, _, _ = oracle.fit(b, opt, track_states=True)
     assert np.array_equal(h["status"], o.status)
     assert np.array_equal(h["n_measured"], o.n_measured)
@@ -102,8 +103,9 @@ def test_host_mixed_fit_within_bars(built, cfg):
     assert pd <= 1e-5 and cd <= 1e-3, (pd, cd)
 
 
+@pytest.mark.parametrize("by_plane", [False, True])
 @pytest.mark.parametrize("precision", [abi.KF_PRECISION_FP64, abi.KF_PRECISION_MIXED])
-def test_host_fit_with_fewer_state_slots_than_planes(built, precision):
+def test_host_fit_with_fewer_state_slots_than_planes(built, precision, by_plane):
     """Config 3 (5-30 planes, 5 % holes) with room for 22 states: tracks
     whose sequence fits 22 slots store their states by plane position, the
     longer ones by measurement index (and fail when they measure more than
@@ -111,7 +113,7 @@ def test_host_fit_with_fewer_state_slots_than_planes(built, precision):
     b = G.generate(3, n_tracks=3000)
     opt = abi.default_options(max_states=22, precision=precision, **W)
     assert (b.seq_count > 22).any() and (b.seq_count <= 22).any()
-    h = host_fit(b, opt)
+    h = host_fit(b, opt, by_plane=by_plane)
     o, _, _, _ = oracle.fit(b, abi.default_options(max_states=22, **W))
     assert np.array_equal(h["status"], o.status)
     assert np.array_equal(h["n_measured"], o.n_measured)
