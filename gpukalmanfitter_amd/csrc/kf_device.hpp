@@ -295,22 +295,21 @@ struct CStep {
     v[kAborter] = kDblMax * (double)dir;
     v[kUser] = value;
   }
-  KFD_INLINE double maxv() const {
-    double m = -kDblMax;
-#pragma unroll
-    for (int i = 0; i < 4; ++i)
-      if (v[i] > m) m = v[i];
-    return m;
-  }
-  KFD_INLINE double minv() const {
+  // s = +1: the minimum of the values, from DBL_MAX with a strict
+  // comparison; s = -1: their maximum, from -DBL_MAX.  One comparison loop
+  // over s * v (negation is exact), so NaNs are skipped and signed zeros kept
+  // as by the two loops of the reference's std::min / std::max.
+  KFD_INLINE double signedMin(double s) const {
     double m = kDblMax;
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
-      if (v[i] < m) m = v[i];
-    return m;
+    for (int i = 0; i < 4; ++i) {
+      const double x = s * v[i];
+      m = (x < m) ? x : m;
+    }
+    return s * m;
   }
   KFD_INLINE void release(int t) {
-    const double r = (dir == 1) ? maxv() : minv();
+    const double r = signedMin(-(double)dir);  // the most permissive value
 #pragma unroll
     for (int i = 0; i < 4; ++i)
       if (i == t) v[i] = r;
@@ -324,7 +323,7 @@ struct CStep {
     v[kAccuracy] = value;
     dir = (value > 0.0) ? 1 : -1;
   }
-  KFD_INLINE double get() const { return (dir == 1) ? minv() : maxv(); }
+  KFD_INLINE double get() const { return signedMin((double)dir); }
 };
 
 }  // namespace kfd

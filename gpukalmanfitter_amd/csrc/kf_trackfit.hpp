@@ -394,6 +394,20 @@ struct TrackFit {
     stepFailed = false;
   }
 
+  // The step-size factor after a rejected trial (EigenStepper.ipp:274-283).
+  // Rejections are rare and the library pow is long: the FP64 kernels call
+  // it out of line, which keeps their propagation loops under the register
+  // budget (the persistent FP64 kernels otherwise spill); in the mixed
+  // kernels the inline copy measured faster.
+  KFD_INLINE static double stepScaleInline(double tol, double err) {
+    double sc = pow(tol / fabs(2.0 * err), 0.25);
+    sc = (0.25 < sc) ? sc : 0.25;
+    return (4.0 < sc) ? 4.0 : sc;
+  }
+  __host__ __device__ __attribute__((noinline)) static double stepScale(double tol, double err) {
+    return stepScaleInline(tol, err);
+  }
+
   // -------------------------------------------------------------- step ----
   // EigenStepper::step, EigenStepper.ipp:226-332, with the transport of
   // transportMatrix (:50-124) applied to J directly.
@@ -446,9 +460,12 @@ struct TrackFit {
       if (err <= a.tolerance) break;
       // std::min(std::max(0.25, x), 4.) with its NaN behaviour: a NaN error
       // estimate (h = DBL_MAX after an actor release) scales by 0.25
-      double sc = pow(a.tolerance / fabs(2.0 * err), 0.25);
-      sc = (0.25 < sc) ? sc : 0.25;
-      sc = (4.0 < sc) ? 4.0 : sc;
+      double sc;
+      if constexpr (kMixed) {
+        sc = stepScaleInline(a.tolerance, err);
+      } else {
+        sc = stepScale(a.tolerance, err);
+      }
       ss.assign(ss.get() * sc);
       const double hn = ss.get();
       if (hn * hn < a.step_size_cutoff * a.step_size_cutoff || nTrials > a.max_rk_trials) {
@@ -1235,8 +1252,10 @@ struct TrackFit {
     cm[KF_SS_MD - KF_SS_CF + 2] = cov[sidx(4, 4)];
     storeS<KF_SS_N - KF_SS_CF>(s, KF_SS_CF, cm);
     if (tsc) saveCheckpoint(k);
-    slotPrev = slotLast;
-    slotLast = s;
+    if constexpr (PS) {
+      slotPrev = slotLast;
+      slotLast = s;
+    }
     ++measured;
     return true;
   }
@@ -1439,8 +1458,9 @@ struct TrackFit {
     const int n = measured;
     double xsn[6], csn[21];
     // last state: smoothed = filtered
-    loadPar(slotLast, KF_SD_XF, xsn);
-    loadFilteredCov(slotLast, csn);
+    const int sl = PS ? slotLast : n - 1;  // its slot
+    loadPar(sl, KF_SD_XF, xsn);
+    loadFilteredCov(sl, csn);
     if (tsf) storeBoundF(n - 1, KF_TS_SMOOTHED, xsn, csn);
     SmoothIn cur;
     if constexpr (!PS) {
@@ -1453,7 +1473,7 @@ struct TrackFit {
       }
     } else {
       int sk = slotPrev;  // slot of state k
-      if (n >= 2) loadSmoothIn(sk, slotLast, cur);
+      if (n >= 2) loadSmoothIn(sk, sl, cur);
       int slot = ((size < a.max_states) ? size : a.max_states) - 1;
       int k = n - 2;
       for (; k >= 0 && slot >= 0; --slot) {
@@ -1568,8 +1588,10 @@ struct TrackFit {
     ci(kCI_measured) = measured;
     ci(kCI_first) = firstPlane;
     ci(kCI_next) = next;
-    ci(kCI_slotLast) = slotLast;
-    ci(kCI_slotPrev) = slotPrev;
+    if constexpr (PS) {
+      ci(kCI_slotLast) = slotLast;
+      ci(kCI_slotPrev) = slotPrev;
+    }
     ci(kCI_flags) = (result ? 1 : 0) | (stepFailed ? 2 : 0) | (navBreak ? 4 : 0) |
                     (terminated ? 8 : 0) | (derivSet ? 16 : 0);
     cd(kCD_q) = q;
@@ -1593,8 +1615,10 @@ struct TrackFit {
     measured = ci(kCI_measured);
     firstPlane = ci(kCI_first);
     next = ci(kCI_next);
-    slotLast = ci(kCI_slotLast);
-    slotPrev = ci(kCI_slotPrev);
+    if constexpr (PS) {
+      slotLast = ci(kCI_slotLast);
+      slotPrev = ci(kCI_slotPrev);
+    }
     const int fl = ci(kCI_flags);
     result = (fl & 1) != 0;
     stepFailed = (fl & 2) != 0;
