@@ -562,8 +562,8 @@ struct PlaneFwd {
       }
     xp[0] = u0[0] * (pos[0] - cc[0]) + u0[1] * (pos[1] - cc[1]) + u0[2] * (pos[2] - cc[2]);
     xp[1] = u1[0] * (pos[0] - cc[0]) + u1[1] * (pos[1] - cc[1]) + u1[2] * (pos[2] - cc[2]);
-    xp[2] = atan2(d[1], d[0]);
-    xp[3] = atan2(rho, d[2]);
+    xp[2] = oolAtan2(d[1], d[0]);
+    xp[3] = oolAtan2(rho, d[2]);
     xp[4] = qop;
     // gain-matrix update
     V00 = v00;
@@ -797,7 +797,8 @@ struct PlaneFwd {
 KFD_INLINE void freeFromBoundVJP(const PlaneFwd &P, const double posb[3], const double db[3],
                                  double pBar, double xfb[5], double cb[3], double Rb[3][3]) {
   const double ph = P.xf[2], th = P.xf[3];
-  const double cp = cos(ph), sp = sin(ph), ct = cos(th), st = sin(th);
+  const SinCos sph = oolSinCos(ph), sth = oolSinCos(th);
+  const double cp = sph.c, sp = sph.s, ct = sth.c, st = sth.s;
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
     xfb[0] += P.u0[k] * posb[k];
@@ -874,8 +875,9 @@ struct MaterialFwd {
       const double u = 2.0 * me * bg * bg, wmax = u / den;
       const double beta2 = 1.0 / (1.0 + (m / p) * (m / p));
       dens = !(bg < 10.0);
-      const double dhalf = dens ? log(bg) + log(a.kPlasma * sqrt(pl.molarElectronDensity) / I) - 0.5 : 0.0;
-      running = 0.5 * log(u / I) + 0.5 * log(wmax / I) - beta2 - dhalf;
+      const double dhalf =
+          dens ? oolLog(bg) + oolLog(a.kPlasma * sqrt(pl.molarElectronDensity) / I) - 0.5 : 0.0;
+      running = 0.5 * oolLog(u / I) + 0.5 * oolLog(wmax / I) - beta2 - dhalf;
       const double dgamma = bg / (gamma * m);
       const double dbeta2 = 2.0 * p * m * m / ((p * p + m * m) * (p * p + m * m));
       drunP = 1.0 / p + 0.5 * (2.0 / p - 2.0 * mfrac * dgamma / den) - dbeta2 - (dens ? 1.0 / p : 0.0);
@@ -883,7 +885,7 @@ struct MaterialFwd {
     }
     if (ms) {
       tt = sqrt(xOverX0 * q2b2);
-      theta0 = a.kHighland * (1.0 / p) * tt * (1.0 + a.kHighlandLog * log(tt));
+      theta0 = a.kHighland * (1.0 / p) * tt * (1.0 + a.kHighlandLog * oolLog(tt));
       rho = sqrt(d[0] * d[0] + d[1] * d[1]);
       dn = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
       sigPhi = theta0 * (dn / rho);
@@ -934,7 +936,7 @@ struct MaterialFwd {
       db[1] += rb * (d[1] / (dn * rho) - dn * d[1] / r3);
       db[2] += rb * (d[2] / (dn * rho));
       // theta0 = kH tt (1 + kHL log tt) / p, tt = sqrt(xOverX0 q2b2)
-      const double dth_dtt = a.kHighland / p * (1.0 + a.kHighlandLog * log(tt) + a.kHighlandLog);
+      const double dth_dtt = a.kHighland / p * (1.0 + a.kHighlandLog * oolLog(tt) + a.kHighlandLog);
       const double ttb = theta0b * dth_dtt;
       pb += theta0b * (-theta0 / p);
       const double xOverX0 = pl.thicknessInX0 * pcor;
@@ -967,7 +969,7 @@ struct TrackAdjoint {
   uint32_t lane;
   double *lds;  // this lane's LDS slots: lds[f * KF_BLOCK]
   float *ldf;   // this lane's float LDS slots: the grid-field gradient of a reversed step
-  TrackFit<double, true> tf;  // recomputes segments from the checkpoints
+  TrackFit<double, true, 1, false, true> tf;  // recomputes segments from the checkpoints
 
   KFD_INLINE TrackAdjoint(const FitArgs &args, const AdjArgs &adj, int64_t idx, int64_t local,
                           double *ckpt, double *stack, uint32_t ln, double *ldsLane,
@@ -1132,8 +1134,9 @@ struct TrackAdjoint {
       const double pF = fabs(1.0 / P.xf[4]);
       double dF[3];
       {
-        const double ct = cos(P.xf[3]), stt = sin(P.xf[3]);
-        const double v0 = cos(P.xf[2]) * stt, v1 = sin(P.xf[2]) * stt, v2 = ct;
+        const SinCos s3 = oolSinCos(P.xf[3]), s2 = oolSinCos(P.xf[2]);
+        const double ct = s3.c, stt = s3.s;
+        const double v0 = s2.c * stt, v1 = s2.s * stt, v2 = ct;
         const double vn = sqrt(v0 * v0 + v1 * v1 + v2 * v2);
         dF[0] = v0 / vn;
         dF[1] = v1 / vn;

@@ -29,7 +29,7 @@ enum { kResumeNone = 0, kResumePreLoop = 1, kResumeInLoop = 2, kResumePostLoop =
 // group run this program on the same data; the free transport matrix of each
 // RK step is built column by column across them, and only the group's first
 // lane writes to memory.
-template <typename S, bool FWD, int L = 1, bool PS = false>
+template <typename S, bool FWD, int L = 1, bool PS = false, bool OOL = false>
 struct TrackFit {
   static constexpr bool kMixed = sizeof(S) < sizeof(double);
   const FitArgs &a;
@@ -322,8 +322,8 @@ struct TrackFit {
   // 41-61) and LineSurface::initJacobianToGlobal (LineSurface.ipp:63-112).
   KFD_INLINE void start(const double pv[6]) {
     const double pabs = fabs(1.0 / pv[4]);
-    const double mom[3] = {pabs * sin(pv[3]) * cos(pv[2]), pabs * sin(pv[3]) * sin(pv[2]),
-                           pabs * cos(pv[3])};
+    const SinCos s3 = sinCosD(pv[3]), s2 = sinCosD(pv[2]);
+    const double mom[3] = {pabs * s3.s * s2.c, pabs * s3.s * s2.s, pabs * s3.c};
     // LineSurface::localToGlobal: centre + (0,0,loc1) + loc0 * |z x mom|
     const double ra[3] = {0.0 - mom[1], mom[0] - 0.0, 0.0};
     const double ran = sqrt(ra[0] * ra[0] + ra[1] * ra[1] + ra[2] * ra[2]);
@@ -392,6 +392,32 @@ struct TrackFit {
     k4d[0] = k4d[1] = k4d[2] = (S)0;
     dtds = (S)0;
     stepFailed = false;
+  }
+
+  // Library sin/cos, atan2 and log: out of line with OOL (the chi2 adjoint's
+  // segment recompute, see oolSinCos), inline in the fit kernels (measured
+  // faster there: the calls on every surface cost more than the registers
+  // they free).
+  KFD_INLINE static SinCos sinCosD(double x) {
+    if constexpr (!OOL) {
+      return SinCos{sin(x), cos(x)};
+    } else {
+      return oolSinCos(x);
+    }
+  }
+  KFD_INLINE static double atan2D(double y, double x) {
+    if constexpr (!OOL) {
+      return atan2(y, x);
+    } else {
+      return oolAtan2(y, x);
+    }
+  }
+  KFD_INLINE static S logS(S x) {
+    if constexpr (!OOL || kMixed) {
+      return log(x);
+    } else {
+      return oolLog(x);
+    }
   }
 
   // The step-size factor after a rejected trial (EigenStepper.ipp:274-283).
@@ -765,8 +791,8 @@ struct TrackFit {
     const double dd[3] = {pos[0] - pl.c[0], pos[1] - pl.c[1], pos[2] - pl.c[2]};
     par[0] = pl.r[0] * dd[0] + pl.r[3] * dd[1] + pl.r[6] * dd[2];
     par[1] = pl.r[1] * dd[0] + pl.r[4] * dd[1] + pl.r[7] * dd[2];
-    par[2] = atan2(mom[1], mom[0]);
-    par[3] = atan2(sqrt(mom[0] * mom[0] + mom[1] * mom[1]), mom[2]);
+    par[2] = atan2D(mom[1], mom[0]);
+    par[3] = atan2D(sqrt(mom[0] * mom[0] + mom[1] * mom[1]), mom[2]);
     const double mn = sqrt(mom[0] * mom[0] + mom[1] * mom[1] + mom[2] * mom[2]);
     par[4] = ((fabs(charge) < 1e-4) ? 1.0 : charge) / mn;
     par[5] = t;
@@ -823,8 +849,8 @@ struct TrackFit {
     const double sgn = (cr[0] * pc[0] + cr[1] * pc[1] + cr[2] * pc[2] < 0.0) ? -1.0 : 1.0;
     par[0] = l0 * sgn;
     par[1] = l3[2];
-    par[2] = atan2(mom[1], mom[0]);
-    par[3] = atan2(sqrt(mom[0] * mom[0] + mom[1] * mom[1]), mom[2]);
+    par[2] = atan2D(mom[1], mom[0]);
+    par[3] = atan2D(sqrt(mom[0] * mom[0] + mom[1] * mom[1]), mom[2]);
     const double mn = sqrt(mom[0] * mom[0] + mom[1] * mom[1] + mom[2] * mom[2]);
     par[4] = ((fabs(charge) < 1e-4) ? 1.0 : charge) / mn;
     par[5] = t;
@@ -837,8 +863,9 @@ struct TrackFit {
   KFD_INLINE void updateFromBound(const DevPlane &pl, const double par[6]) {
 #pragma unroll
     for (int k = 0; k < 3; ++k) pos[k] = (pl.r[3 * k] * par[0] + pl.r[3 * k + 1] * par[1]) + pl.c[k];
-    const double cT = cos(par[3]), sT = sin(par[3]);
-    const double d0 = cos(par[2]) * sT, d1 = sin(par[2]) * sT, d2 = cT;
+    const SinCos s3 = sinCosD(par[3]), s2 = sinCosD(par[2]);
+    const double cT = s3.c, sT = s3.s;
+    const double d0 = s2.c * sT, d1 = s2.s * sT, d2 = cT;
     const double idn = 1.0 / sqrt(d0 * d0 + d1 * d1 + d2 * d2);
     dir[0] = d0 * idn;
     dir[1] = d1 * idn;
@@ -1038,20 +1065,20 @@ struct TrackFit {
       S dhalf = (S)0;
       if (!(betaGamma < (S)10)) {
         const S plasma = (S)a.kPlasma * sqrt(Ne);
-        dhalf = log(betaGamma) + log(plasma / I) - (S)0.5;
+        dhalf = logS(betaGamma) + logS(plasma / I) - (S)0.5;
       }
       const S u = (S)2 * kMe * betaGamma * betaGamma;
       const S mfrac = kMe / mass;
       const S nom = (S)2 * kMe * betaGamma * betaGamma;
       const S den = (S)1 + (S)2 * gamma * mfrac + mfrac * mfrac;
       const S wmax = nom / den;
-      const S running = (S)0.5 * log(u / I) + (S)0.5 * log(wmax / I) - beta2 - dhalf;
+      const S running = (S)0.5 * logS(u / I) + (S)0.5 * logS(wmax / I) - beta2 - dhalf;
       Eloss = eps * running;
     }
     if (a.ms) {
       const S tt = sqrt(xOverX0 * q2OverBeta2);
       const S theta0 =
-          (S)a.kHighland * fabs(qOverP / qq) * tt * ((S)1 + (S)a.kHighlandLog * log(tt));
+          (S)a.kHighland * fabs(qOverP / qq) * tt * ((S)1 + (S)a.kHighlandLog * logS(tt));
       const S dn = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
       const S sigmaPhi = theta0 * (dn / sqrt(d[0] * d[0] + d[1] * d[1]));
       vPhi = sigmaPhi * sigmaPhi;
