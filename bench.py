@@ -204,8 +204,10 @@ def launch_figures(prec, phase, n, ms):
     traffic = TRAFFIC_BYTES_PER_LAUNCH.get((prec, phase))
     if traffic is not None:
         traffic *= n / TRAFFIC_TRACKS
-    return {"kernel": "kf_fit_kernel<%s, %s, true, false>" % ("float" if mixed else "double",
-                                                            "true" if phase == "forward" else "false"),
+    # the launch of the default schedule on a constant field (the last
+    # template argument 1 = kFieldConst)
+    return {"kernel": "kf_fit_kernel<%s, %s, true, false, 1, false, 1>" % (
+                "float" if mixed else "double", "true" if phase == "forward" else "false"),
             "kernel_ms": ms, "flops_per_track": f, "fp32_flops_per_track": g,
             "t_roof_ms": t_roof * 1e3, "tflop_per_s": f * n / t / 1e12,
             "vector_peak_tflop_per_s": f * n / t_roof / 1e12, "vector_frac": t_roof / t,

@@ -118,21 +118,26 @@ struct DevField {
   double iw[3];  // 1 / w: cell index and fraction by multiplication
 };
 
-// Field lookup in double.  Grid: trilinear, last axis first (the reference's
-// interpolation_impl.hpp:98-131), positions clamped to the box.
-KFD_INLINE void fieldAt(const DevField &f, double x, double y, double z, double B[3]) {
-  if (!f.grid) {
-    if (f.scale == 1.0) {
-      B[0] = f.b[0];
-      B[1] = f.b[1];
-      B[2] = f.b[2];
-    } else {
-      B[0] = f.scale * f.b[0];
-      B[1] = f.scale * f.b[1];
-      B[2] = f.scale * f.b[2];
-    }
-    return;
+// Field kinds a kernel can be built for: either (chosen per lookup by
+// DevField::grid), the constant field only, the grid only.
+constexpr int kFieldAny = 0, kFieldConst = 1, kFieldGrid = 2;
+
+// Field lookup in double.  Constant field: the vector times the scale.
+KFD_INLINE void fieldConst(const DevField &f, double B[3]) {
+  if (f.scale == 1.0) {
+    B[0] = f.b[0];
+    B[1] = f.b[1];
+    B[2] = f.b[2];
+  } else {
+    B[0] = f.scale * f.b[0];
+    B[1] = f.scale * f.b[1];
+    B[2] = f.scale * f.b[2];
   }
+}
+
+// Grid: trilinear, last axis first (the reference's
+// interpolation_impl.hpp:98-131), positions clamped to the box.
+KFD_INLINE void fieldGrid(const DevField &f, double x, double y, double z, double B[3]) {
   const double p[3] = {x, y, z};
   int idx[3];
   double fr[3];
@@ -171,6 +176,14 @@ KFD_INLINE void fieldAt(const DevField &f, double x, double y, double z, double 
     B[k] = (1.0 - fr[0]) * y0 + fr[0] * y1;
     if (f.scale != 1.0) B[k] = f.scale * B[k];
   }
+}
+
+// Either kind, chosen by f.grid at run time.
+KFD_INLINE void fieldAt(const DevField &f, double x, double y, double z, double B[3]) {
+  if (!f.grid)
+    fieldConst(f, B);
+  else
+    fieldGrid(f, x, y, z, B);
 }
 
 // Field and its spatial derivative dB[k][axis] = dB_k / dx_axis (the chi2

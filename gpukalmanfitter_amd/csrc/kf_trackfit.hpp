@@ -29,7 +29,7 @@ enum { kResumeNone = 0, kResumePreLoop = 1, kResumeInLoop = 2, kResumePostLoop =
 // group run this program on the same data; the free transport matrix of each
 // RK step is built column by column across them, and only the group's first
 // lane writes to memory.
-template <typename S, bool FWD, int L = 1, bool PS = false, bool OOL = false>
+template <typename S, bool FWD, int L = 1, bool PS = false, bool OOL = false, int FK = kFieldAny>
 struct TrackFit {
   static constexpr bool kMixed = sizeof(S) < sizeof(double);
   const FitArgs &a;
@@ -289,7 +289,16 @@ struct TrackFit {
   KFD_INLINE static constexpr int jidx(int r, int c) { return ((r < 4) ? r : 4) * 5 + c; }
 
   // -------------------------------------------------------------- field ----
-  KFD_INLINE void field(const double x[3], double B[3]) const { fieldAt(a.field, x[0], x[1], x[2], B); }
+  // FK: the field kind the kernel was built for (kFieldConst and
+  // kFieldGrid leave the other kind's code out of the step loop)
+  KFD_INLINE void field(const double x[3], double B[3]) const {
+    if constexpr (FK == kFieldConst)
+      fieldConst(a.field, B);
+    else if constexpr (FK == kFieldGrid)
+      fieldGrid(a.field, x[0], x[1], x[2], B);
+    else
+      fieldAt(a.field, x[0], x[1], x[2], B);
+  }
 
   // ------------------------------------------------ Jacobian init (plane) ----
   // PlaneSurface::initJacobianToGlobal, PlaneSurface.ipp:122-156
