@@ -204,9 +204,9 @@ def launch_figures(prec, phase, n, ms):
     traffic = TRAFFIC_BYTES_PER_LAUNCH.get((prec, phase))
     if traffic is not None:
         traffic *= n / TRAFFIC_TRACKS
-    # the launch of the default schedule on a constant field (the last
-    # template argument 1 = kFieldConst)
-    return {"kernel": "kf_fit_kernel<%s, %s, true, false, 1, false, 1>" % (
+    # the launch of the default schedule on a constant field without track
+    # states (the last two template arguments: kFieldConst = 1, no recording)
+    return {"kernel": "kf_fit_kernel<%s, %s, true, false, 1, false, 1, false>" % (
                 "float" if mixed else "double", "true" if phase == "forward" else "false"),
             "kernel_ms": ms, "flops_per_track": f, "fp32_flops_per_track": g,
             "t_roof_ms": t_roof * 1e3, "tflop_per_s": f * n / t / 1e12,

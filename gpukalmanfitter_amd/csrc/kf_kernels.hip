@@ -27,15 +27,15 @@ struct ChunkBases {
 
 // Fits the track at position pos (pass FWD of the two kernels); `rank` is
 // the lane's index in the track's group of L lanes.
-template <typename S, bool FWD, int L, bool PS, int FK>
+template <typename S, bool FWD, int L, bool PS, int FK, bool REC>
 KFD_INLINE void fitPosition(const FitArgs &a, int64_t local, uint32_t rank,
                             const DevPlane *planes) {
   const ChunkBases<S> cb(a, local);
   if (local >= a.count) return;
   const int64_t pos = a.first + local;
   const int64_t i = a.order ? (int64_t)a.order[pos] : pos;
-  TrackFit<S, FWD, L, PS, false, FK> tf(a, i, cb.d, cb.s, cb.f, cb.c, (uint32_t)(local & 63),
-                                        local);
+  TrackFit<S, FWD, L, PS, false, FK, REC> tf(a, i, cb.d, cb.s, cb.f, cb.c,
+                                             (uint32_t)(local & 63), local);
   tf.setRank(rank);
   tf.pls = planes;
   if constexpr (FWD)
@@ -57,10 +57,13 @@ KFD_INLINE void fitPosition(const FitArgs &a, int64_t local, uint32_t rank,
 // position, a wave covers 64 / L positions of one 64-position store chunk.
 // PS: store slots by plane position (kf_schedule.store_by_plane; one lane
 // per track).
-// FK: the field kind (kFieldConst or kFieldGrid: the launcher picks the one
-// of the batch's field, so the step loop carries no code of the other kind;
-// kFieldAny for plane tables read from global memory).
-template <typename S, bool FWD, bool LDS, bool PERS, int L, bool PS, int FK>
+// FK, REC: the field kind and whether track states or checkpoints are
+// written.  The launcher builds on (kFieldConst, false) or (kFieldGrid,
+// false) for the batch's field when neither is asked for, so the step loop
+// carries no code of the other field kind and the filter none of the
+// recording; otherwise, and for plane tables read from global memory, on
+// (kFieldAny, true).
+template <typename S, bool FWD, bool LDS, bool PERS, int L, bool PS, int FK, bool REC>
 __global__ void __launch_bounds__(KF_BLOCK) kf_fit_kernel(FitArgs args) {
   static_assert(L == 1 || !PERS, "lane groups run on the one-shot grid");
   static_assert(L == 1 || !PS, "store slots by plane with one lane per track");
@@ -77,7 +80,7 @@ __global__ void __launch_bounds__(KF_BLOCK) kf_fit_kernel(FitArgs args) {
   const uint32_t lane = threadIdx.x & 63u;
   if constexpr (!PERS) {
     const int64_t t = (int64_t)blockIdx.x * KF_BLOCK + threadIdx.x;
-    fitPosition<S, FWD, L, PS, FK>(args, t / L, (uint32_t)(t % L), planes);
+    fitPosition<S, FWD, L, PS, FK, REC>(args, t / L, (uint32_t)(t % L), planes);
   } else {
     const uint32_t nchunks = (uint32_t)((args.count + 63) >> 6);
     const uint32_t grab = (uint32_t)args.chunks_per_grab;
@@ -89,7 +92,7 @@ __global__ void __launch_bounds__(KF_BLOCK) kf_fit_kernel(FitArgs args) {
       if (c0 >= nchunks) break;
       const uint32_t c1 = (c0 + grab < nchunks) ? c0 + grab : nchunks;
       for (uint32_t c = c0; c < c1; ++c)
-        fitPosition<S, FWD, 1, PS, FK>(args, (int64_t)c * 64 + lane, 0u, planes);
+        fitPosition<S, FWD, 1, PS, FK, REC>(args, (int64_t)c * 64 + lane, 0u, planes);
     }
   }
 }
@@ -191,32 +194,35 @@ extern "C" hipError_t kfd_launch_fit(const kfd::FitArgs *args, int precision, hi
   const bool lds = args->n_planes <= KF_LDS_PLANES;
   const bool fp64 = precision == KF_PRECISION_FP64;
   const bool gridField = args->field.grid != 0;
-#define KFD_LAUNCH2(S_, FWD_, LDS_, PERS_, L_, PS_, FK_)                                      \
-  hipLaunchKernelGGL((kfd::kf_fit_kernel<S_, FWD_, LDS_, PERS_, L_, PS_, FK_>), grid, block, \
-                     0, stream, *args)
-#define KFD_LAUNCH1(S_, FWD_, LDS_, FK_)                             \
+  const bool rec = args->tsf != nullptr || args->tsc != nullptr;
+#define KFD_LAUNCH2(S_, FWD_, LDS_, PERS_, L_, PS_, FK_, REC_)                             \
+  hipLaunchKernelGGL((kfd::kf_fit_kernel<S_, FWD_, LDS_, PERS_, L_, PS_, FK_, REC_>), grid, \
+                     block, 0, stream, *args)
+#define KFD_LAUNCH1(S_, FWD_, LDS_, FK_, REC_)                       \
   do {                                                              \
     if (pers && ps)                                                 \
-      KFD_LAUNCH2(S_, FWD_, LDS_, true, 1, true, FK_);              \
+      KFD_LAUNCH2(S_, FWD_, LDS_, true, 1, true, FK_, REC_);        \
     else if (pers)                                                  \
-      KFD_LAUNCH2(S_, FWD_, LDS_, true, 1, false, FK_);             \
+      KFD_LAUNCH2(S_, FWD_, LDS_, true, 1, false, FK_, REC_);       \
     else if (lpt == 1 && ps)                                        \
-      KFD_LAUNCH2(S_, FWD_, LDS_, false, 1, true, FK_);             \
+      KFD_LAUNCH2(S_, FWD_, LDS_, false, 1, true, FK_, REC_);       \
     else if (lpt == 1)                                              \
-      KFD_LAUNCH2(S_, FWD_, LDS_, false, 1, false, FK_);            \
+      KFD_LAUNCH2(S_, FWD_, LDS_, false, 1, false, FK_, REC_);      \
     else if (lpt == 2)                                              \
-      KFD_LAUNCH2(S_, FWD_, LDS_, false, 2, false, FK_);            \
+      KFD_LAUNCH2(S_, FWD_, LDS_, false, 2, false, FK_, REC_);      \
     else                                                            \
-      KFD_LAUNCH2(S_, FWD_, LDS_, false, 4, false, FK_);            \
+      KFD_LAUNCH2(S_, FWD_, LDS_, false, 4, false, FK_, REC_);      \
   } while (0)
-#define KFD_LAUNCH(S_, FWD_)                          \
-  do {                                                \
-    if (lds && gridField)                             \
-      KFD_LAUNCH1(S_, FWD_, true, kfd::kFieldGrid);   \
-    else if (lds)                                     \
-      KFD_LAUNCH1(S_, FWD_, true, kfd::kFieldConst);  \
-    else                                              \
-      KFD_LAUNCH1(S_, FWD_, false, kfd::kFieldAny);   \
+#define KFD_LAUNCH(S_, FWD_)                                 \
+  do {                                                       \
+    if (lds && !rec && gridField)                            \
+      KFD_LAUNCH1(S_, FWD_, true, kfd::kFieldGrid, false);   \
+    else if (lds && !rec)                                    \
+      KFD_LAUNCH1(S_, FWD_, true, kfd::kFieldConst, false);  \
+    else if (lds)                                            \
+      KFD_LAUNCH1(S_, FWD_, true, kfd::kFieldAny, true);     \
+    else                                                     \
+      KFD_LAUNCH1(S_, FWD_, false, kfd::kFieldAny, true);    \
   } while (0)
   if (fp64)
     KFD_LAUNCH(double, true);

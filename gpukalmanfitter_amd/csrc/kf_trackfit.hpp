@@ -29,7 +29,8 @@ enum { kResumeNone = 0, kResumePreLoop = 1, kResumeInLoop = 2, kResumePostLoop =
 // group run this program on the same data; the free transport matrix of each
 // RK step is built column by column across them, and only the group's first
 // lane writes to memory.
-template <typename S, bool FWD, int L = 1, bool PS = false, bool OOL = false, int FK = kFieldAny>
+template <typename S, bool FWD, int L = 1, bool PS = false, bool OOL = false, int FK = kFieldAny,
+          bool REC = true>
 struct TrackFit {
   static constexpr bool kMixed = sizeof(S) < sizeof(double);
   const FitArgs &a;
@@ -140,10 +141,13 @@ struct TrackFit {
   // the launch, so consecutive lanes use consecutive slots
   int64_t cpos;
 
+  // REC = false: a build for fits that record neither track states nor
+  // checkpoints (both stores are then null at compile time, and their
+  // writes are not in the code)
   KFD_INLINE TrackFit(const FitArgs &args, int64_t idx, double *d_base, S *s_base, double *f_base,
                       double *c_base, uint32_t ln, int64_t carry_slot = 0)
-      : a(args), i(idx), tsd(d_base), tss(s_base), tsf(f_base), tsc(c_base), pls(args.planes),
-        lane(ln), cpos(carry_slot) {}
+      : a(args), i(idx), tsd(d_base), tss(s_base), tsf(REC ? f_base : nullptr),
+        tsc(REC ? c_base : nullptr), pls(args.planes), lane(ln), cpos(carry_slot) {}
 
   // ------------------------------------------------------------ storage ----
   // Element f of state k: base[(k * F + f) * 64 + lane] -- a 32-bit lane
