@@ -433,18 +433,17 @@ struct TrackFit {
     }
   }
 
-  // The step-size factor after a rejected trial (EigenStepper.ipp:274-283).
-  // Rejections are rare and the library pow is long: the FP64 kernels call
-  // it out of line, which keeps their propagation loops under the register
-  // budget (the persistent FP64 kernels otherwise spill); in the mixed
-  // kernels the inline copy measured faster.
-  KFD_INLINE static double stepScaleInline(double tol, double err) {
-    double sc = pow(tol / fabs(2.0 * err), 0.25);
+  // The step-size factor after a rejected trial (EigenStepper.ipp:274-283):
+  // pow(tolerance / |2 err|, 0.25) clamped to [0.25, 4].  The fourth root is
+  // taken as two square roots (each correctly rounded: within an ulp of the
+  // pow, the same NaN, 0 and infinity cases); the inlined library pow was
+  // ~200 instructions in the step loop and its temporaries raised the loop's
+  // register peak (an out-of-line call of it, or of this, measured within
+  // 1 % in the FP64 kernels and slower in the mixed ones).
+  KFD_INLINE static double stepScale(double tol, double err) {
+    double sc = sqrt(sqrt(tol / fabs(2.0 * err)));
     sc = (0.25 < sc) ? sc : 0.25;
     return (4.0 < sc) ? 4.0 : sc;
-  }
-  __host__ __device__ __attribute__((noinline)) static double stepScale(double tol, double err) {
-    return stepScaleInline(tol, err);
   }
 
   // -------------------------------------------------------------- step ----
@@ -499,13 +498,7 @@ struct TrackFit {
       if (err <= a.tolerance) break;
       // std::min(std::max(0.25, x), 4.) with its NaN behaviour: a NaN error
       // estimate (h = DBL_MAX after an actor release) scales by 0.25
-      double sc;
-      if constexpr (kMixed) {
-        sc = stepScaleInline(a.tolerance, err);
-      } else {
-        sc = stepScale(a.tolerance, err);
-      }
-      ss.assign(ss.get() * sc);
+      ss.assign(ss.get() * stepScale(a.tolerance, err));
       const double hn = ss.get();
       if (hn * hn < a.step_size_cutoff * a.step_size_cutoff || nTrials > a.max_rk_trials) {
         stepFailed = true;
