@@ -228,7 +228,7 @@ int kf_fit(kf_ctx *ctx, const kf_tracks *tracks, const kf_options *opt,
  * schedule (no length sort, work queue or graph replay).  Returns
  * KF_E_INVALID when the working storage of one 64-track slice,
  * 64 x (max_states x (512 B in FP64, 304 B mixed, + 960 B with track
- * states) + 264 B), does not fit the workspace (max_states: opt->max_states,
+ * states) + 272 B), does not fit the workspace (max_states: opt->max_states,
  * else the plane-table size; in FP64 without track states up to 8,191).  Calls on
  * one device share its workspace: calls that overlap in time on one GPU
  * (different streams or host threads) race and must be serialised by the
