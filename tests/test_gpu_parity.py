@@ -247,7 +247,7 @@ def test_hit_order_and_duplicate_hits(torch_cuda, precision):
         check_parity(g, o)
     else:
         pd, cd, _ = compare_fit(g, o)
-        assert pd <= TOL_MIXED_PARAMS and cd <= TOL_MIXED_COV, (pd, cd)
+        assert pd <= 2 * TOL_MIXED_PARAMS and cd <= 2 * TOL_MIXED_COV, (pd, cd)
 
 
 def test_empty_batch_and_single_track(torch_cuda):
@@ -587,3 +587,28 @@ def test_autotuner_on_ragged_batch(torch_cuda):
     for k in ("params", "cov", "status"):
         assert np.array_equal(got[k], ref[k]), k
     kf.close()
+
+
+@pytest.mark.parametrize("cfg", [1, 3, 5])
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_builds_with_and_without_records_agree(torch_cuda, cfg, precision):
+    """A fit that asks for track states runs the build that writes them (and
+    takes either field kind), one that does not runs the build for the
+    batch's field kind without the record writes: the two run the same
+    operations and agree to rounding in FP64; in the mixed mode each is
+    within the bars of the oracle, so they are within twice the bars of each
+    other.  Statuses and measurement counts are equal."""
+    from types import SimpleNamespace
+    b = G.generate(cfg, n_tracks=2000)
+    opts = dict(W, precision=precision)
+    plain = run_gpu(b, **opts)
+    rec = run_gpu(b, track_states=True, **opts)
+    assert np.array_equal(plain["status"], rec["status"])
+    assert np.array_equal(plain["n_measured"], rec["n_measured"])
+    ref = SimpleNamespace(status=rec["status"], params=rec["params"], cov=rec["cov"],
+                          chi2=rec["chi2"])
+    pd, cd, c2 = compare_fit(plain, ref)
+    if precision == "fp64":
+        assert max(pd, cd, c2) <= 1e-10, (pd, cd, c2)
+    else:
+        assert pd <= 2 * TOL_MIXED_PARAMS and cd <= 2 * TOL_MIXED_COV, (pd, cd)
