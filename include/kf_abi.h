@@ -164,7 +164,10 @@ typedef struct kf_results {
   int32_t *ndf;       /* [n_tracks] 2 * (measured - outliers) */
   int32_t *n_measured;/* [n_tracks] KalmanFitterResult::measurementStates */
   uint32_t *status;   /* [n_tracks] KF_STATUS_* */
-  double *track_states; /* optional [max_states][KF_TS_STRIDE][n_tracks] */
+  double *track_states; /* optional [max_states][KF_TS_STRIDE][n_tracks]; a fit
+                           that asks for them runs the kernel build that writes them,
+                           whose results agree with a fit without them to rounding
+                           (FP64 ~1e-12), not bit for bit */
 } kf_results;
 
 /* How kf_fit lays the batch onto the GPU (this build's; the reference
