@@ -5,6 +5,7 @@ tests/parity.py) on every output, per-track status bits are identical, and at
 the full 1,048,576-track size the fit is deterministic and its pulls are
 N(0, 1).  Inputs are the seeded workloads of gpukalmanfitter_amd.configs.
 """
+import ctypes
 import math
 
 import numpy as np
@@ -612,3 +613,45 @@ def test_builds_with_and_without_records_agree(torch_cuda, cfg, precision):
         assert max(pd, cd, c2) <= 1e-10, (pd, cd, c2)
     else:
         assert pd <= 2 * TOL_MIXED_PARAMS and cd <= 2 * TOL_MIXED_COV, (pd, cd)
+
+
+def _padded_planes(planes, extra):
+    """The plane table followed by `extra` planes that no sequence uses
+    (copies of the last plane, 10 m further along x, with new geo ids)."""
+    n = len(planes)
+    out = (abi.kf_plane * (n + extra))()
+    ctypes.memmove(out, planes, ctypes.sizeof(abi.kf_plane) * n)
+    gmax = max(planes[i].geo_id for i in range(n))
+    for j in range(extra):
+        p = out[n + j]
+        ctypes.memmove(ctypes.byref(p), ctypes.byref(planes[n - 1]), ctypes.sizeof(abi.kf_plane))
+        p.center[0] = planes[n - 1].center[0] + 10000.0 + j
+        p.geo_id = gmax + 1 + j
+    return out
+
+
+@pytest.mark.parametrize("cfg", [3, 5])
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_plane_table_larger_than_lds(torch_cuda, cfg, precision):
+    """A plane table of more planes than the kernels copy to LDS (256) is
+    read from global memory by the build that takes either field kind: the
+    fits match the oracle at the usual bars and the fit of the unpadded table
+    to rounding (FP64)."""
+    import dataclasses
+    b = G.generate(cfg, n_tracks=2000)
+    bp = dataclasses.replace(b, planes=_padded_planes(b.planes, 300))
+    assert len(bp.planes) > 256
+    opts = dict(W, precision=precision)
+    g = run_gpu(bp, **opts)
+    o, _, _, _ = oracle.fit(bp, c_options(**W))
+    if precision == "fp64":
+        check_parity(g, o)
+        ref = run_gpu(b, **opts)
+        from types import SimpleNamespace
+        pd, cd, c2 = compare_fit(g, SimpleNamespace(status=ref["status"], params=ref["params"],
+                                                    cov=ref["cov"], chi2=ref["chi2"]))
+        assert max(pd, cd, c2) <= 1e-10, (pd, cd, c2)
+    else:
+        assert np.array_equal(g["status"], o.status)
+        pd, cd, _ = compare_fit(g, o)
+        assert pd <= TOL_MIXED_PARAMS and cd <= TOL_MIXED_COV, (pd, cd)
