@@ -1294,6 +1294,20 @@ struct TrackFit {
   }
 
   // -------------------------------------------------------------- smoother ----
+  // The diagonal scale of spdSolve, ~1 / sqrt(c).  Any positive scale gives
+  // the same solution in exact arithmetic (the factorisation works on the
+  // scaled diagonal as computed, not on an assumed 1), so the GPU takes the
+  // hardware's reciprocal square root without refinement (one instruction
+  // instead of a correctly rounded square root and division); NaN for c < 0
+  // and infinity for c = 0 as 1 / sqrt(c).
+  KFD_INLINE static double scaleOf(double c) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_rsq(c);
+#else
+    return 1.0 / sqrt(c);  // host build of the fit program
+#endif
+  }
+
   // X = C^-1 M for a symmetric positive-definite 6x6 C (packed) and six
   // right-hand sides, in place on M: LDL^T factorisation of the
   // diagonally scaled matrix S C S (S = diag(C)^-1/2, unit diagonal, so no
@@ -1306,7 +1320,7 @@ struct TrackFit {
   KFD_INLINE static void spdSolve(const double C[21], double M[6][6]) {
     double sc[6];
 #pragma unroll
-    for (int k = 0; k < 6; ++k) sc[k] = 1.0 / sqrt(C[sidx(k, k)]);
+    for (int k = 0; k < 6; ++k) sc[k] = scaleOf(C[sidx(k, k)]);
     // W[r][k] = Lo[r][k] D[k] (r > k), Lo below the diagonal, Dinv = 1 / D
     double Lo[6][6], W[6][6], Dinv[6];
 #pragma unroll
