@@ -64,8 +64,8 @@ HBM_PEAK_GBS = 8000.0
 # calibration (profiles/kf_fit_config2_1M_tracks.md).
 TRAFFIC_TRACKS = 1048576
 TRAFFIC_BYTES_PER_LAUNCH = {
-    ("fp64", "forward"): 2.194e9 + 5.662e9, ("fp64", "backward"): 5.557e9 + 0.231e9,
-    ("mixed", "forward"): 2.121e9 + 3.481e9, ("mixed", "backward"): 3.510e9 + 0.231e9,
+    ("fp64", "forward"): 2.318e9 + 5.662e9, ("fp64", "backward"): 5.555e9 + 0.231e9,
+    ("mixed", "forward"): 2.265e9 + 3.481e9, ("mixed", "backward"): 3.510e9 + 0.231e9,
 }
 PARAM_NAMES = ("loc0", "loc1", "phi", "theta", "qop", "t")
 
